@@ -1,0 +1,356 @@
+"""Python binding of the MI355X PatchMatch-MVS library (libhp3d_pm.so).
+
+Thin ctypes layer over the C ABI declared in include/hp3d_pm.h and
+include/hp3d_scene.h.  Nothing here computes: every solve runs in the HIP
+kernels of csrc/.  If the library is not built, importing this package
+raises, so no caller can silently fall back to a CPU path.
+
+The directory name contains a hyphen; load it with load_package() from
+__graft_entry__.py (or importlib) under the module name
+``hemiphoto3d_private_amd``.
+"""
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libhp3d_pm.so")
+
+MAX_SRC = 8
+OK = 0
+ERR_INVALID_ARG, ERR_OOM_BUDGET, ERR_DEVICE, ERR_CANCELLED = -1, -2, -3, -4
+ERR_NOT_FOUND, ERR_UNSUPPORTED, ERR_IO, ERR_AGAIN = -5, -6, -7, -8
+FMT_U8, FMT_F32 = 0, 1
+STORE_F32, STORE_F16 = 0, 1
+CTX_TIMING = 1
+SCENE_PLANE, SCENE_SPHERE_GROUND = 0, 1
+
+_f32p = ctypes.POINTER(ctypes.c_float)
+_f64p = ctypes.POINTER(ctypes.c_double)
+_u8p = ctypes.POINTER(ctypes.c_uint8)
+
+
+class HP3DError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"hp3d error {code}: {msg}")
+        self.code = code
+
+
+class Opts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("stream", ctypes.c_void_p), ("flags", ctypes.c_int)]
+
+
+class ViewDesc(ctypes.Structure):
+    _fields_ = [
+        ("ref", ctypes.c_uint64), ("src", ctypes.c_uint64 * MAX_SRC), ("num_src", ctypes.c_int),
+        ("K_ref_inv", ctypes.c_double * 9),
+        ("A", (ctypes.c_double * 9) * MAX_SRC), ("b", (ctypes.c_double * 3) * MAX_SRC),
+        ("depth_min", ctypes.c_float), ("depth_max", ctypes.c_float),
+        ("radius", ctypes.c_int), ("stride", ctypes.c_int), ("iterations", ctypes.c_int),
+        ("top_k", ctypes.c_int), ("seed", ctypes.c_uint64),
+        ("init_depth", _f32p), ("init_normal", _f32p),
+    ]
+
+
+class ViewOut(ctypes.Structure):
+    _fields_ = [("depth", _f32p), ("normal", _f32p), ("cost", _f32p), ("confidence", _f32p),
+                ("on_device", ctypes.c_int)]
+
+
+class Stats(ctypes.Structure):
+    _fields_ = [("ms_total", ctypes.c_double), ("ms_init", ctypes.c_double),
+                ("ms_cost", ctypes.c_double), ("ms_propagate", ctypes.c_double),
+                ("n_propagate", ctypes.c_int), ("ms_finalize", ctypes.c_double),
+                ("taps", ctypes.c_double), ("hbm_in_use", ctypes.c_size_t),
+                ("hbm_budget", ctypes.c_size_t)]
+
+
+class Scene(ctypes.Structure):
+    _fields_ = [("kind", ctypes.c_int), ("seed", ctypes.c_uint64), ("octaves", ctypes.c_int),
+                ("tex_freq", ctypes.c_double), ("plane_point", ctypes.c_double * 3),
+                ("plane_normal", ctypes.c_double * 3), ("sphere_center", ctypes.c_double * 3),
+                ("sphere_radius", ctypes.c_double), ("ground_y", ctypes.c_double),
+                ("backdrop_z", ctypes.c_double)]
+
+
+class Camera(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("K", ctypes.c_double * 9),
+                ("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3)]
+
+
+# Every exported symbol of the C ABI, with (restype, argtypes).
+SIGNATURES = {
+    "hp3d_pm_version": (ctypes.c_char_p, []),
+    "hp3d_pm_create": (ctypes.c_int, [ctypes.POINTER(Opts), ctypes.POINTER(ctypes.c_void_p)]),
+    "hp3d_pm_destroy": (ctypes.c_int, [ctypes.c_void_p]),
+    "hp3d_pm_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "hp3d_pm_image_upload": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_int,
+                                            ctypes.c_int, ctypes.c_size_t, ctypes.c_int,
+                                            ctypes.c_int, ctypes.POINTER(ctypes.c_uint64)]),
+    "hp3d_pm_image_release": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "hp3d_pm_image_info": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64,
+                                          ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_int),
+                                          ctypes.POINTER(ctypes.c_size_t)]),
+    "hp3d_pm_compose": (ctypes.c_int, [_f64p] * 8),
+    "hp3d_pm_invert3": (ctypes.c_int, [_f64p, _f64p]),
+    "hp3d_pm_run_view": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc),
+                                        ctypes.POINTER(ViewOut)]),
+    "hp3d_pm_init_hypotheses": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc),
+                                               _f32p, _f32p]),
+    "hp3d_pm_cost_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc), _f32p, _f32p,
+                                         _f32p]),
+    "hp3d_pm_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    "hp3d_pm_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t,
+                                            ctypes.POINTER(ctypes.c_void_p)]),
+    "hp3d_pm_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
+    "hp3d_pm_device_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
+                                           ctypes.c_size_t, ctypes.c_int]),
+    "hp3d_pfm_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p]),
+    "hp3d_pfm_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                     ctypes.POINTER(ctypes.c_int), _f32p, ctypes.c_size_t]),
+    "hp3d_scene_render": (ctypes.c_int, [ctypes.POINTER(Scene), ctypes.POINTER(Camera),
+                                         ctypes.c_double, ctypes.c_uint64, _u8p, _f32p, _f32p,
+                                         ctypes.c_int]),
+}
+
+
+def build():
+    """Compile the HIP kernels and host runtime into libhp3d_pm.so (in-tree)."""
+    subprocess.check_call(["make", "-s", "-C", HERE, f"-j{min(8, os.cpu_count() or 1)}"])
+
+
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError(f"{LIB_PATH} is not built; run __graft_entry__.build() "
+                              "(make -C hemiphoto3d-private_amd)")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _p(a, t):
+    return a.ctypes.data_as(t)
+
+
+def _check(code, ctx=None):
+    if code != OK:
+        msg = lib().hp3d_pm_last_error(ctx).decode() if ctx else ""
+        raise HP3DError(code, msg)
+
+
+# ---------------------------------------------------------------------------
+# Host math
+# ---------------------------------------------------------------------------
+def invert3(M):
+    M = np.ascontiguousarray(M, dtype=np.float64).reshape(9)
+    out = np.zeros(9)
+    _check(lib().hp3d_pm_invert3(_p(M, _f64p), _p(out, _f64p)))
+    return out.reshape(3, 3)
+
+
+def compose(K_ref, R_ref, t_ref, K_src, R_src, t_src):
+    """FP64 plane-homography terms (A, b) of one source view."""
+    args = [np.ascontiguousarray(a, dtype=np.float64).reshape(-1)
+            for a in (K_ref, R_ref, t_ref, K_src, R_src, t_src)]
+    A = np.zeros(9)
+    b = np.zeros(3)
+    _check(lib().hp3d_pm_compose(*[_p(a, _f64p) for a in args], _p(A, _f64p), _p(b, _f64p)))
+    return A.reshape(3, 3), b
+
+
+def pfm_save(path, depth):
+    depth = np.ascontiguousarray(depth, dtype=np.float32)
+    H, W = depth.shape
+    _check(lib().hp3d_pfm_save(os.fsencode(path), W, H, _p(depth, _f32p)))
+
+
+def pfm_load(path):
+    W, H = ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().hp3d_pfm_load(os.fsencode(path), ctypes.byref(W), ctypes.byref(H), None, 0))
+    out = np.zeros((H.value, W.value), dtype=np.float32)
+    _check(lib().hp3d_pfm_load(os.fsencode(path), ctypes.byref(W), ctypes.byref(H),
+                               _p(out, _f32p), out.size))
+    return out
+
+
+def render(scene, camera, noise_sigma=0.0, noise_seed=0, threads=0, gt=True):
+    """Render (image uint8 (H,W), gt_depth (H,W) or None, gt_normal (H,W,3) or None)."""
+    H, W = camera.height, camera.width
+    img = np.zeros((H, W), dtype=np.uint8)
+    gd = np.zeros((H, W), dtype=np.float32) if gt else None
+    gn = np.zeros((H, W, 3), dtype=np.float32) if gt else None
+    rc = lib().hp3d_scene_render(ctypes.byref(scene), ctypes.byref(camera), float(noise_sigma),
+                                 int(noise_seed), _p(img, _u8p),
+                                 _p(gd, _f32p) if gt else None, _p(gn, _f32p) if gt else None,
+                                 int(threads))
+    if rc != 0:
+        raise HP3DError(ERR_INVALID_ARG, "scene render failed")
+    return img, gd, gn
+
+
+# ---------------------------------------------------------------------------
+# Device context
+# ---------------------------------------------------------------------------
+class Context:
+    """One hp3d_pm_ctx: images, state buffers and a HIP stream on one device."""
+
+    def __init__(self, device=0, hbm_budget=0, timing=False):
+        o = Opts(device, hbm_budget, None, CTX_TIMING if timing else 0)
+        h = ctypes.c_void_p()
+        code = lib().hp3d_pm_create(ctypes.byref(o), ctypes.byref(h))
+        if code != OK:
+            raise HP3DError(code, "hp3d_pm_create failed (no usable HIP device?)")
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().hp3d_pm_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def last_error(self):
+        return lib().hp3d_pm_last_error(self.h).decode()
+
+    def upload(self, image, store=STORE_F32):
+        """Upload an (H, W) uint8 DN image or float32 gray image in [0, 1]."""
+        a = np.ascontiguousarray(image)
+        if a.dtype == np.uint8:
+            fmt = FMT_U8
+        elif a.dtype == np.float32:
+            fmt = FMT_F32
+        else:
+            raise TypeError("image must be uint8 or float32")
+        H, W = a.shape
+        out = ctypes.c_uint64(0)
+        _check(lib().hp3d_pm_image_upload(self.h, a.ctypes.data, W, H, a.strides[0], fmt,
+                                          store, ctypes.byref(out)), self.h)
+        return out.value
+
+    def release(self, img):
+        _check(lib().hp3d_pm_image_release(self.h, img), self.h)
+
+    def image_info(self, img):
+        W, H, st, nb = ctypes.c_int(), ctypes.c_int(), ctypes.c_int(), ctypes.c_size_t()
+        _check(lib().hp3d_pm_image_info(self.h, img, ctypes.byref(W), ctypes.byref(H),
+                                        ctypes.byref(st), ctypes.byref(nb)), self.h)
+        return W.value, H.value, st.value, nb.value
+
+    def stats(self):
+        s = Stats()
+        _check(lib().hp3d_pm_get_stats(self.h, ctypes.byref(s)), self.h)
+        return {f: getattr(s, f) for f, _ in Stats._fields_}
+
+    def run_view(self, desc, shape, with_normal=True, with_cost=True, with_confidence=False):
+        H, W = shape
+        depth = np.zeros((H, W), dtype=np.float32)
+        normal = np.zeros((H, W, 3), dtype=np.float32) if with_normal else None
+        cost = np.zeros((H, W), dtype=np.float32) if with_cost else None
+        conf = np.zeros((H, W), dtype=np.float32) if with_confidence else None
+        out = ViewOut(_p(depth, _f32p), _p(normal, _f32p) if with_normal else None,
+                      _p(cost, _f32p) if with_cost else None,
+                      _p(conf, _f32p) if with_confidence else None, 0)
+        _check(lib().hp3d_pm_run_view(self.h, ctypes.byref(desc), ctypes.byref(out)), self.h)
+        res = {"depth": depth, "normal": normal, "cost": cost}
+        if with_confidence:
+            res["confidence"] = conf
+        return res
+
+    def device_alloc(self, nbytes):
+        """Device buffer owned by the context (counted against its HBM budget)."""
+        p = ctypes.c_void_p()
+        _check(lib().hp3d_pm_device_alloc(self.h, int(nbytes), ctypes.byref(p)), self.h)
+        return p.value
+
+    def device_free(self, ptr):
+        _check(lib().hp3d_pm_device_free(self.h, ctypes.c_void_p(ptr)), self.h)
+
+    def download(self, ptr, shape, dtype=np.float32):
+        out = np.empty(shape, dtype=dtype)
+        _check(lib().hp3d_pm_device_copy(self.h, out.ctypes.data, ctypes.c_void_p(ptr),
+                                         out.nbytes, 1), self.h)
+        return out
+
+    def upload_bytes(self, ptr, array):
+        a = np.ascontiguousarray(array)
+        _check(lib().hp3d_pm_device_copy(self.h, ctypes.c_void_p(ptr), a.ctypes.data, a.nbytes, 0),
+               self.h)
+
+    def run_view_device(self, desc, depth_ptr, normal_ptr=0, cost_ptr=0):
+        """Solve into caller-owned device buffers (raw device addresses)."""
+        out = ViewOut(ctypes.cast(depth_ptr, _f32p), ctypes.cast(normal_ptr or None, _f32p),
+                      ctypes.cast(cost_ptr or None, _f32p), None, 1)
+        _check(lib().hp3d_pm_run_view(self.h, ctypes.byref(desc), ctypes.byref(out)), self.h)
+
+    def init_hypotheses(self, desc, shape):
+        H, W = shape
+        depth = np.zeros((H, W), dtype=np.float32)
+        normal = np.zeros((H, W, 3), dtype=np.float32)
+        _check(lib().hp3d_pm_init_hypotheses(self.h, ctypes.byref(desc), _p(depth, _f32p),
+                                             _p(normal, _f32p)), self.h)
+        return depth, normal
+
+    def cost_eval(self, desc, depth, normal):
+        depth = np.ascontiguousarray(depth, dtype=np.float32)
+        normal = np.ascontiguousarray(normal, dtype=np.float32)
+        cost = np.zeros(depth.shape, dtype=np.float32)
+        _check(lib().hp3d_pm_cost_eval(self.h, ctypes.byref(desc), _p(depth, _f32p),
+                                       _p(normal, _f32p), _p(cost, _f32p)), self.h)
+        return cost
+
+
+def make_desc(ref, srcs, K_ref_inv, A, b, depth_min, depth_max, radius=5, stride=1,
+              iterations=0, top_k=3, seed=42, init=None):
+    """Build a ViewDesc.  ref/srcs are image handles; A, b lists of FP64 terms.
+
+    init: optional (depth (H,W), normal (H,W,3)) float32 host arrays replacing
+    the random initialisation; the arrays are kept alive on the descriptor.
+    """
+    d = ViewDesc()
+    d.ref = ref
+    for i, s in enumerate(srcs):
+        d.src[i] = s
+    d.num_src = len(srcs)
+    Ki = np.asarray(K_ref_inv, dtype=np.float64).reshape(9)
+    for i in range(9):
+        d.K_ref_inv[i] = Ki[i]
+    for s in range(len(srcs)):
+        a = np.asarray(A[s], dtype=np.float64).reshape(9)
+        bb = np.asarray(b[s], dtype=np.float64).reshape(3)
+        for i in range(9):
+            d.A[s][i] = a[i]
+        for i in range(3):
+            d.b[s][i] = bb[i]
+    d.depth_min, d.depth_max = float(depth_min), float(depth_max)
+    d.radius, d.stride, d.iterations, d.top_k = int(radius), int(stride), int(iterations), int(top_k)
+    d.seed = int(seed)
+    if init is not None:
+        idp = np.ascontiguousarray(init[0], dtype=np.float32)
+        inn = np.ascontiguousarray(init[1], dtype=np.float32)
+        d._keep = (idp, inn)
+        d.init_depth = _p(idp, _f32p)
+        d.init_normal = _p(inn, _f32p)
+    return d

@@ -1,0 +1,60 @@
+// pm_device.h -- types shared by the host runtime and the gfx950 kernels of
+// the PatchMatch-MVS path.
+//
+// Data layout in HBM (one reference view being solved):
+//   * images: every uploaded image keeps two planes, both shifted by -128 DN
+//     so that window sums stay small (the NCC is invariant to the shift):
+//       plain[y][x]        = DN - 128                 (reference windows)
+//       coef [y][x] (x<W-1, y<H-1) = (a, b-a, c-a, a-b-c+d), a = plain[y][x],
+//         b = plain[y][x+1], c = plain[y+1][x], d = plain[y+1][x+1]
+//     so that a bilinear sample at (x+fx, y+fy) is
+//       a + fx*(b-a) + fy*((c-a) + fx*(a-b-c+d))   -- one load, three FMAs.
+//     Element type is float (F32 storage) or _Float16 (F16 storage).  For
+//     8-bit input every coefficient is an integer in [-510, 510], so F16
+//     storage is lossless.
+//   * hypotheses: two compacted colour planes (red: (x+y) even, black: odd),
+//     each H x Wc with Wc = (W+1)/2.  Pixel (x, y) of colour c sits at
+//     column i = x >> 1 of plane c.  hyp = (depth, nx, ny, nz) as float4,
+//     cost as float.  A half-pass reads the other plane only.
+#pragma once
+#include <stdint.h>
+
+namespace hp3d {
+
+constexpr int kMaxSrc = 8;
+constexpr int kMaxTopK = 8;
+constexpr float kShiftDN = 128.0f;
+
+enum Store : int { kStoreF32 = 0, kStoreF16 = 1 };
+
+// Homography terms of one source, kept in FP64: the warp of a window centre
+// is formed in FP64 and only the small tap offsets in FP32, so the warped
+// coordinates carry no cancellation error from large pixel coordinates.
+struct DevSrc {
+    double A[9];   // A = K_s R_rel K_ref^-1, row-major
+    double b[3];   // b = K_s t_rel
+};
+
+struct DevView {
+    const void* ref_plain;              // reference plain plane
+    const void* src_coef[kMaxSrc];      // source bilinear-coefficient planes
+    int pitch_plain;                    // elements per row of a plain plane
+    int pitch_coef;                     // elements (texels) per row of a coef plane
+    float kinv[9];                      // K_ref^-1 (row-major), hypothesis arithmetic
+    double kinv_d[9];                   // K_ref^-1 for the cost
+    DevSrc src[kMaxSrc];
+    int W, H, S, Wc;
+    float dmin, dmax;
+    float den_eps;                      // NCC variance guard in stored units^2
+    int top_k;
+    float4* hyp[2];                     // colour planes (depth, nx, ny, nz)
+    float* cost[2];
+};
+
+// Philox4x32-10 (Salmon et al., SC'11).  Used on host and device.
+constexpr uint32_t kPhiloxM0 = 0xD2511F53u, kPhiloxM1 = 0xCD9E8D57u;
+constexpr uint32_t kPhiloxW0 = 0x9E3779B9u, kPhiloxW1 = 0xBB67AE85u;
+constexpr uint32_t kDomainInit = 0x494E4954u;    // "INIT"
+constexpr uint32_t kDomainRefine = 0x52464E45u;  // "RFNE"
+
+}  // namespace hp3d

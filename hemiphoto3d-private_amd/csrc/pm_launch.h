@@ -1,0 +1,26 @@
+// pm_launch.h -- host-callable launchers of the kernels in pm_kernels.hip.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pm_device.h"
+
+namespace hp3d {
+
+enum PassMode : int { kPassCost = 0, kPassPropagate = 1 };
+
+bool window_supported(int radius, int stride);
+hipError_t launch_init(const DevView& v, uint64_t seed, hipStream_t st);
+hipError_t launch_finalize(const DevView& v, float* depth, float* normal, float* cost, float* conf,
+                           hipStream_t st);
+hipError_t launch_import(const DevView& v, const float* depth, const float* normal,
+                         hipStream_t st);
+hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_pitch, int W,
+                           int H, int store, void* plain, int pitch_plain, void* coef,
+                           int pitch_coef, hipStream_t st);
+// mode kPassCost: cost of the current hypotheses of colour `parity`;
+// mode kPassPropagate: one half-pass of colour `parity` at `iteration`.
+hipError_t launch_pass(const DevView& v, int store, int radius, int stride, int mode,
+                       int iteration, int parity, uint64_t pass_seed, hipStream_t st);
+
+}  // namespace hp3d

@@ -1,0 +1,101 @@
+// pm_runtime.h -- host-side context and helpers behind the C ABI.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "../../include/hp3d_pm.h"
+#include "pm_device.h"
+#include "pm_launch.h"
+
+namespace hp3d {
+
+// One uploaded image: plain + bilinear-coefficient planes (pm_device.h).
+struct Image {
+    int W = 0, H = 0, store = 0;
+    int pitch_plain = 0, pitch_coef = 0;   // elements
+    void* plain = nullptr;
+    void* coef = nullptr;
+    size_t plain_bytes = 0, coef_bytes = 0;
+    float den_scale = 1.0f;                // stored-unit^2 per gray^2
+};
+
+struct DevBuf {
+    void* ptr = nullptr;
+    size_t bytes = 0;
+};
+
+enum Phase : int { kPhaseStart = 0, kPhaseInit, kPhaseCost, kPhasePropagate, kPhaseFinalize };
+
+struct Mark {
+    int phase;
+    hipEvent_t ev;
+};
+
+}  // namespace hp3d
+
+struct hp3d_pm_ctx {
+    int device = 0;
+    int flags = 0;
+    hipStream_t stream = nullptr;
+    bool own_stream = false;
+    size_t hbm_budget = 0, hbm_in_use = 0;
+    std::string last_error;
+    std::unordered_map<hp3d_img, hp3d::Image> images;
+    hp3d_img next_handle = 0;
+    std::unordered_map<void*, size_t> user_bufs;   // hp3d_pm_device_alloc
+
+    // per-view working set, grown on demand and reused across views
+    hp3d::DevBuf hyp, cost, out_depth, out_normal, out_cost, in_depth, in_normal, staging;
+
+    hp3d_pm_stats stats{};
+    std::vector<hp3d::Mark> marks;
+
+    int fail(int code, const char* fmt, ...) __attribute__((format(printf, 3, 4)));
+    int hip_fail(hipError_t e, const char* what);
+    int dev_alloc(void** p, size_t bytes, const char* what);
+    void dev_free(void* p, size_t bytes);
+    int grow(hp3d::DevBuf& b, size_t bytes, const char* what);
+    const hp3d::Image* find(hp3d_img h) const;
+    void mark(int phase);
+    void collect_marks(int n_prop);
+
+    void release_buf(hp3d::DevBuf& b)
+    {
+        dev_free(b.ptr, b.bytes);
+        b.ptr = nullptr;
+        b.bytes = 0;
+    }
+    void release_all()
+    {
+        for (auto& kv : images) {
+            dev_free(kv.second.plain, kv.second.plain_bytes);
+            dev_free(kv.second.coef, kv.second.coef_bytes);
+        }
+        images.clear();
+        for (auto& kv : user_bufs) dev_free(kv.first, kv.second);
+        user_bufs.clear();
+        for (hp3d::DevBuf* b : {&hyp, &cost, &out_depth, &out_normal, &out_cost, &in_depth,
+                                &in_normal, &staging})
+            release_buf(*b);
+    }
+};
+
+namespace hp3d {
+
+int validate_desc(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image** ref,
+                  const Image* srcs[kMaxSrc]);
+void fill_dev_view(DevView& v, const hp3d_view_desc* d, const Image* ref, const Image* const* srcs);
+double algorithmic_taps(int W, int H, int S, int radius, int stride, int iterations);
+uint64_t pass_seed(uint64_t seed, int iteration, int parity);
+
+int solve_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
+               const Image* const* srcs, const hp3d_view_out* out);
+int init_only(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
+              const Image* const* srcs, float* depth, float* normal);
+int cost_only(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
+              const Image* const* srcs, const float* depth, const float* normal, float* cost);
+
+}  // namespace hp3d

@@ -1,0 +1,166 @@
+/*
+ * hp3d_pm.h -- C ABI of the MI355X (gfx950) PatchMatch multi-view-stereo path.
+ *
+ * This is the boundary the reference's C++ API sits on.  The reference's
+ * entry points and what replaces each (paths relative to the reference):
+ *
+ *   runPatchMatchMVS(images, cameras, sfm, config)
+ *       src/dense_recon/patchmatch_mvs.h:39-44, .cu:1028-1388
+ *       -> hp3d_pm_image_upload (textures, .cu:1042-1055)
+ *        + hp3d_pm_compose      (precomputeSourceViews, .cu:847-884, in FP64)
+ *        + hp3d_pm_run_view     (per-view body, .cu:1076-1258)
+ *        + hp3d_pm_geom_filter  (filter driver, .cu:1267-1384)
+ *   init_depth_normal_kernel launch     .cu:890-902   -> hp3d_pm_init_hypotheses
+ *   compute_ncc_cost_kernel launch      .cu:904-934   -> hp3d_pm_cost_eval
+ *   selectSourceViews(sfm, n)           patchmatch_mvs.h:32-35 -> hp3d_select_source_views
+ *   saveDepthMapPFM / loadDepthMapPFM   patchmatch_mvs.h:59,62 -> hp3d_pfm_save / hp3d_pfm_load
+ *
+ * Conventions
+ *   - Every function returns an int status: HP3D_OK (0) or a negative
+ *     hp3d_status.  Nothing here calls exit(); the reference's CUDA_CHECK
+ *     exits the process (src/utils/cuda_utils.cuh:7-15).
+ *   - A context is used by one host thread at a time; contexts are
+ *     independent.  All device work of a context runs on its stream.
+ *   - Matrices are row-major doubles.  Cameras map world to camera:
+ *     P_cam = R P_world + t (include/types.h:55-74).
+ *   - Outputs: depth H*W float (z-depth), normal H*W*3 float (camera frame),
+ *     cost H*W float in [0, 2]; confidence = 1 - cost/2 (.cu:1246-1252).
+ */
+#ifndef HP3D_PM_H
+#define HP3D_PM_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HP3D_PM_MAX_SRC 8   /* MAX_SRC_VIEWS, patchmatch_mvs.cu:18 */
+#define HP3D_PM_MAX_TOPK 8
+
+typedef enum {
+    HP3D_OK = 0,
+    HP3D_ERR_INVALID_ARG = -1,
+    HP3D_ERR_OOM_BUDGET = -2,   /* request exceeds the context's HBM budget */
+    HP3D_ERR_DEVICE = -3,       /* a HIP call failed */
+    HP3D_ERR_CANCELLED = -4,
+    HP3D_ERR_NOT_FOUND = -5,    /* unknown handle or job */
+    HP3D_ERR_UNSUPPORTED = -6,  /* e.g. window radius/stride not built */
+    HP3D_ERR_IO = -7,
+    HP3D_ERR_AGAIN = -8         /* job not finished yet */
+} hp3d_status;
+
+typedef enum { HP3D_FMT_U8 = 0, HP3D_FMT_F32 = 1 } hp3d_pixel_format;
+typedef enum { HP3D_STORE_F32 = 0, HP3D_STORE_F16 = 1 } hp3d_storage;
+
+#define HP3D_CTX_TIMING 1   /* record HIP events around every launch */
+
+typedef struct hp3d_pm_ctx hp3d_pm_ctx;
+typedef uint64_t hp3d_img;
+
+typedef struct {
+    int device;          /* HIP device ordinal */
+    size_t hbm_budget;   /* bytes of device memory the context may hold; 0 = 90% of free */
+    void* stream;        /* hipStream_t to run on; NULL = the context creates one */
+    int flags;           /* HP3D_CTX_* */
+} hp3d_pm_opts;
+
+/* One reference view to solve (the body of the per-view loop,
+ * patchmatch_mvs.cu:1076-1258). */
+typedef struct {
+    hp3d_img ref;
+    hp3d_img src[HP3D_PM_MAX_SRC];
+    int num_src;                         /* 1..8 */
+    double K_ref_inv[9];                 /* K_ref^-1 */
+    double A[HP3D_PM_MAX_SRC][9];        /* from hp3d_pm_compose */
+    double b[HP3D_PM_MAX_SRC][3];
+    float depth_min, depth_max;          /* MVSViewParams, patchmatch_mvs.h:23-28 */
+    int radius;                          /* window half size 1..5 (.cu:1065-1070) */
+    int stride;                          /* window stride 1 or 2 (1 = reference) */
+    int iterations;                      /* PipelineConfig::mvs_iterations */
+    int top_k;                           /* NCC_TOP_K = 3 in the reference */
+    uint64_t seed;                       /* view seed; reference: 42 + 1e6*ref_id (.cu:1106) */
+    const float* init_depth;             /* optional host hypotheses replacing the random */
+    const float* init_normal;            /*   init (H*W and H*W*3); both or neither */
+} hp3d_view_desc;
+
+typedef struct {
+    float* depth;        /* H*W, required */
+    float* normal;       /* H*W*3, nullable */
+    float* cost;         /* H*W, nullable */
+    float* confidence;   /* H*W, nullable: 1 - cost/2 */
+    int on_device;       /* 1: the pointers are device memory of the context's device */
+} hp3d_view_out;
+
+typedef struct {
+    double ms_total;       /* last run: first launch to last completion (device events) */
+    double ms_init;
+    double ms_cost;        /* initial-cost launches */
+    double ms_propagate;   /* sum over half-pass launches */
+    int n_propagate;       /* half-pass launches in the last run */
+    double ms_finalize;
+    double taps;           /* algorithmic taps of the last run (upper bound) */
+    size_t hbm_in_use;     /* bytes currently held by the context */
+    size_t hbm_budget;
+} hp3d_pm_stats;
+
+const char* hp3d_pm_version(void);
+
+int hp3d_pm_create(const hp3d_pm_opts* opts, hp3d_pm_ctx** out);
+int hp3d_pm_destroy(hp3d_pm_ctx* ctx);
+/* Message of the last failed call on ctx ("" if none).  Owned by ctx. */
+const char* hp3d_pm_last_error(const hp3d_pm_ctx* ctx);
+
+/* Upload one gray image.  fmt U8: 8-bit DN; fmt F32: gray in [0,1] as in
+ * ImageData::gray (include/types.h:17).  pitch in bytes.  Replaces
+ * createGrayscaleTexture (patchmatch_mvs.cu:789-814). */
+int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int width, int height,
+                         size_t pitch, int fmt, int store, hp3d_img* out);
+int hp3d_pm_image_release(hp3d_pm_ctx* ctx, hp3d_img img);
+int hp3d_pm_image_info(const hp3d_pm_ctx* ctx, hp3d_img img, int* width, int* height,
+                       int* store, size_t* bytes);
+
+/* FP64 composition of the plane-homography terms for one source:
+ * R_rel = R_s R_ref^T, t_rel = t_s - R_rel t_ref, A = K_s R_rel K_ref^-1,
+ * b = K_s t_rel (precomputeSourceViews, patchmatch_mvs.cu:847-884). */
+int hp3d_pm_compose(const double K_ref[9], const double R_ref[9], const double t_ref[3],
+                    const double K_src[9], const double R_src[9], const double t_src[3],
+                    double A[9], double b[3]);
+/* FP64 3x3 inverse (Eigen's K.inverse(), patchmatch_mvs.cu:834). */
+int hp3d_pm_invert3(const double M[9], double Minv[9]);
+
+/* Solve one view: random init (or desc->init_*), initial cost, then
+ * desc->iterations red/black half-passes.  Blocks until out is written. */
+int hp3d_pm_run_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, const hp3d_view_out* out);
+
+/* Random initial hypotheses only (the reference's mvs_iterations = 0 maps,
+ * minus the cost).  depth H*W, normal H*W*3, host memory. */
+int hp3d_pm_init_hypotheses(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, float* depth,
+                            float* normal);
+
+/* Cost of caller-given hypotheses (host depth H*W, normal H*W*3) -> host cost H*W. */
+int hp3d_pm_cost_eval(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, const float* depth,
+                      const float* normal, float* cost);
+
+int hp3d_pm_get_stats(const hp3d_pm_ctx* ctx, hp3d_pm_stats* out);
+
+/* Device buffers owned by the context and counted against its HBM budget,
+ * for callers without a HIP runtime of their own (the reference keeps its
+ * device state in DeviceBuffer<T>, src/utils/cuda_utils.cuh:26-57).
+ * hp3d_pm_device_copy kind: 0 host->device, 1 device->host, 2 device->device;
+ * it completes before returning. */
+int hp3d_pm_device_alloc(hp3d_pm_ctx* ctx, size_t bytes, void** dptr);
+int hp3d_pm_device_free(hp3d_pm_ctx* ctx, void* dptr);
+int hp3d_pm_device_copy(hp3d_pm_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+
+/* Portable FloatMap depth I/O, byte-compatible with saveDepthMapPFM /
+ * loadDepthMapPFM (patchmatch_mvs.cu:990-1022): "Pf\n<W> <H>\n-1.0\n", rows
+ * bottom to top.  hp3d_pfm_load with data == NULL only reports the size. */
+int hp3d_pfm_save(const char* path, int width, int height, const float* depth);
+int hp3d_pfm_load(const char* path, int* width, int* height, float* data, size_t capacity);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* HP3D_PM_H */

@@ -1,0 +1,177 @@
+"""CPU tests of the product's host side (no GPU compute calls).
+
+  - libhp3d_pm.so loads and exports every function include/*.h declares;
+  - FP64 composition (hp3d_pm_compose) agrees with the oracle's and with a
+    pose-free closed form;
+  - PFM files are byte-compatible with saveDepthMapPFM (patchmatch_mvs.cu:990-1004);
+  - the scene renderer's ground truth matches closed forms;
+  - error paths return codes instead of exiting.
+"""
+import math
+import os
+import re
+import struct
+
+import numpy as np
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_functions():
+    names = set()
+    for h in ("hp3d_pm.h", "hp3d_scene.h"):
+        text = open(os.path.join(ROOT, "include", h)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(hp3d_\w+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_symbol(pkg):
+    lib = pkg.lib()
+    names = declared_functions()
+    assert len(names) >= 15
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    # the Python binding declares a signature for every one of them
+    assert names <= set(pkg.SIGNATURES), sorted(names - set(pkg.SIGNATURES))
+
+
+def test_version_string(pkg):
+    assert b"gfx950" in pkg.lib().hp3d_pm_version()
+
+
+def test_compose_matches_oracle(pkg, orc):
+    rng = np.random.default_rng(1)
+    for _ in range(20):
+        Kr = np.array([[rng.uniform(100, 3000), 0, rng.uniform(50, 2000)],
+                       [0, rng.uniform(100, 3000), rng.uniform(50, 1500)], [0, 0, 1]])
+        q = rng.normal(size=(2, 4))
+        Rs = []
+        for qq in q:
+            w, x, y, z = qq / np.linalg.norm(qq)
+            Rs.append(np.array([[1 - 2 * (y * y + z * z), 2 * (x * y - z * w), 2 * (x * z + y * w)],
+                                [2 * (x * y + z * w), 1 - 2 * (x * x + z * z), 2 * (y * z - x * w)],
+                                [2 * (x * z - y * w), 2 * (y * z + x * w), 1 - 2 * (x * x + y * y)]]))
+        tr, ts = rng.normal(size=3), rng.normal(size=3)
+        A1, b1 = pkg.compose(Kr, Rs[0], tr, Kr, Rs[1], ts)
+        A2, b2 = orc.compose(Kr, Rs[0], tr, Kr, Rs[1], ts)
+        np.testing.assert_allclose(A1, A2, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(b1, b2, rtol=1e-12, atol=1e-12)
+
+
+def test_compose_pure_translation_gives_disparity(pkg):
+    f, W, H, B, z = 1500.0, 1600, 1200, 0.3, 5.0
+    K = np.array([[f, 0, W / 2], [0, f, H / 2], [0, 0, 1]])
+    A, b = pkg.compose(K, np.eye(3), np.zeros(3), K, np.eye(3), np.array([-B, 0, 0]))
+    # fronto-parallel plane at depth z: w = A p + b (n.K^-1 p)/(n.P), n = (0,0,-1)
+    for x, y in ((0, 0), (800, 600), (1599, 1199)):
+        p = np.array([x, y, 1.0])
+        factor = 1.0 / z                       # (n.K^-1 p) / (n.P) = -1 / (-z)
+        w = A @ p + b * factor
+        assert w[0] / w[2] == pytest.approx(x - f * B / z, abs=1e-9)
+        assert w[1] / w[2] == pytest.approx(y, abs=1e-9)
+
+
+def test_invert3(pkg):
+    M = np.array([[2.0, 1, 0], [0, 3, 1], [1, 0, 4]])
+    np.testing.assert_allclose(pkg.invert3(M) @ M, np.eye(3), atol=1e-14)
+    with pytest.raises(pkg.HP3DError):
+        pkg.invert3(np.zeros((3, 3)))
+
+
+def test_pfm_round_trip_and_byte_layout(pkg, tmp_path):
+    depth = np.random.default_rng(2).uniform(0.5, 12, size=(7, 5)).astype(np.float32)
+    depth[0, 0] = 0.0
+    p = str(tmp_path / "d.pfm")
+    pkg.pfm_save(p, depth)
+    raw = open(p, "rb").read()
+    header = b"Pf\n5 7\n-1.0\n"
+    assert raw.startswith(header)
+    body = np.frombuffer(raw[len(header):], dtype="<f4").reshape(7, 5)
+    assert np.array_equal(body[::-1], depth)           # rows bottom to top
+    back = pkg.pfm_load(p)
+    assert back.dtype == np.float32 and np.array_equal(back, depth)   # bit-exact (PFMSaveLoad)
+
+
+def test_pfm_load_big_endian_and_errors(pkg, tmp_path):
+    d = np.arange(6, dtype=np.float32).reshape(2, 3) + 0.25
+    p = str(tmp_path / "be.pfm")
+    with open(p, "wb") as fh:
+        fh.write(b"Pf\n3 2\n1.0\n")
+        fh.write(d[::-1].astype(">f4").tobytes())
+    assert np.array_equal(pkg.pfm_load(p), d)
+    bad = str(tmp_path / "bad.pfm")
+    open(bad, "wb").write(b"PF\n3 2\n-1.0\n")
+    with pytest.raises(pkg.HP3DError):
+        pkg.pfm_load(bad)
+    with pytest.raises(pkg.HP3DError):
+        pkg.pfm_load(str(tmp_path / "missing.pfm"))
+
+
+def test_plane_scene_ground_truth_closed_form(scenes, c1):
+    H, W = c1.shape
+    y = np.arange(H)[:, None].astype(np.float64)
+    gt = 5.0 / (1.0 - math.tan(math.radians(20)) * (y - 120) / 300)
+    np.testing.assert_allclose(c1.gt_depth, np.broadcast_to(gt, (H, W)), rtol=2e-7)
+    n = np.array([0.0, math.sin(math.radians(20)), -math.cos(math.radians(20))])
+    np.testing.assert_allclose(c1.gt_normal.reshape(-1, 3), np.broadcast_to(n, (H * W, 3)), atol=1e-6)
+    # textured, inside the 32 + 192 v range of the noise-free renderer
+    assert c1.ref.dtype == np.uint8 and c1.ref.std() > 10
+    assert c1.ref.min() >= 32 and c1.ref.max() <= 224
+
+
+def test_sphere_scene_ground_truth_ray_sphere(scenes):
+    job = scenes.sphere_job(160, 120, 2, iterations=0)
+    Kinv = np.linalg.inv(job.K)
+    H, W = job.shape
+    for x, y in ((80, 60), (70, 50), (95, 68)):
+        r = Kinv @ np.array([x, y, 1.0])
+        c = np.array([0.0, 0.0, 6.0])
+        a, b, cc = r @ r, -2 * r @ c, c @ c - 1.5 ** 2
+        t = (-b - math.sqrt(b * b - 4 * a * cc)) / (2 * a)
+        assert job.gt_depth[y, x] == pytest.approx(t * r[2], rel=1e-6)
+        nrm = (t * r - c) / 1.5
+        np.testing.assert_allclose(job.gt_normal[y, x], nrm, atol=1e-5)
+    assert np.all((job.gt_depth >= 2.0) & (job.gt_depth <= 12.0))
+    # backdrop and ground fill everything the sphere does not cover
+    assert job.gt_depth[0, 0] == pytest.approx(11.5, rel=1e-6)
+
+
+def test_scene_render_is_deterministic_and_seeded(scenes):
+    a = scenes.sphere_job(96, 72, 2, scene_seed=3, iterations=0)
+    b = scenes.sphere_job(96, 72, 2, scene_seed=3, iterations=0)
+    c = scenes.sphere_job(96, 72, 2, scene_seed=4, iterations=0)
+    assert np.array_equal(a.ref, b.ref) and np.array_equal(a.srcs[1], b.srcs[1])
+    assert not np.array_equal(a.ref, c.ref)
+
+
+def test_arc_angles_layout(scenes):
+    assert scenes.arc_angles(4) == [-15.0, -7.5, 7.5, 15.0]
+    assert scenes.arc_angles(6) == [-15.0, -10.0, -5.0, 5.0, 10.0, 15.0]
+    for S in range(1, 9):
+        a = scenes.arc_angles(S)
+        assert len(a) == S and len(set(a)) == S and max(abs(x) for x in a) == 15.0 or S == 1
+
+
+def test_context_creation_fails_cleanly_without_device(pkg):
+    import ctypes
+    # with no visible GPU the call must return an error code, not exit
+    if os.environ.get("HIP_VISIBLE_DEVICES") == "" or not _has_gpu():
+        h = ctypes.c_void_p()
+        code = pkg.lib().hp3d_pm_create(ctypes.byref(pkg.Opts(0, 0, None, 0)), ctypes.byref(h))
+        assert code == pkg.ERR_DEVICE
+    else:
+        pytest.skip("a GPU is visible")
+
+
+def _has_gpu():
+    return os.path.exists("/dev/kfd") and os.access("/dev/kfd", os.R_OK | os.W_OK)
+
+
+def test_null_context_arguments(pkg):
+    lib = pkg.lib()
+    assert lib.hp3d_pm_destroy(None) == pkg.ERR_INVALID_ARG
+    assert lib.hp3d_pm_run_view(None, None, None) == pkg.ERR_INVALID_ARG
+    assert lib.hp3d_pfm_save(None, 1, 1, None) == pkg.ERR_INVALID_ARG
