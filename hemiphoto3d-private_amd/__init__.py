@@ -60,6 +60,53 @@ class ViewOut(ctypes.Structure):
                 ("on_device", ctypes.c_int)]
 
 
+class BatchOpts(ctypes.Structure):
+    _fields_ = [("variant", ctypes.c_int), ("use_graph", ctypes.c_int)]
+
+
+NUM_VARIANTS = 6
+JOB_QUEUED, JOB_RUNNING, JOB_DONE, JOB_FAILED, JOB_CANCELLED = 0, 1, 2, 3, 4
+
+
+class ServerOpts(ctypes.Structure):
+    _fields_ = [("device", ctypes.c_int), ("hbm_budget", ctypes.c_size_t),
+                ("max_jobs", ctypes.c_int), ("radius", ctypes.c_int), ("stride", ctypes.c_int),
+                ("store", ctypes.c_int), ("variant", ctypes.c_int)]
+
+
+class JobDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("num_src", ctypes.c_int),
+                ("ref", _u8p), ("src", _u8p * MAX_SRC),
+                ("K_ref_inv", ctypes.c_double * 9),
+                ("A", (ctypes.c_double * 9) * MAX_SRC), ("b", (ctypes.c_double * 3) * MAX_SRC),
+                ("depth_min", ctypes.c_float), ("depth_max", ctypes.c_float),
+                ("iterations", ctypes.c_int), ("top_k", ctypes.c_int), ("seed", ctypes.c_uint64),
+                ("out_depth", _f32p), ("out_normal", _f32p), ("out_cost", _f32p),
+                ("fail_at_step", ctypes.c_int)]
+
+
+class JobStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("error", ctypes.c_int), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double), ("steps", ctypes.c_int)]
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("steps", ctypes.c_longlong), ("busy_ms", ctypes.c_double),
+                ("wall_s", ctypes.c_double), ("submitted", ctypes.c_longlong),
+                ("done", ctypes.c_longlong), ("failed", ctypes.c_longlong),
+                ("cancelled", ctypes.c_longlong), ("rejected", ctypes.c_longlong),
+                ("taps", ctypes.c_double), ("mpix_iters", ctypes.c_double),
+                ("arena_bytes", ctypes.c_size_t), ("arena_peak", ctypes.c_size_t),
+                ("max_active", ctypes.c_int)]
+
+
+class GeomView(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("depth", _f32p),
+                ("normal", _f32p), ("confidence", _f32p), ("K", ctypes.c_double * 9),
+                ("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3), ("num_src", ctypes.c_int),
+                ("src", ctypes.c_int * MAX_SRC)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("ms_total", ctypes.c_double), ("ms_init", ctypes.c_double),
                 ("ms_cost", ctypes.c_double), ("ms_propagate", ctypes.c_double),
@@ -104,6 +151,30 @@ SIGNATURES = {
     "hp3d_pm_cost_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc), _f32p, _f32p,
                                          _f32p]),
     "hp3d_pm_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(Stats)]),
+    "hp3d_pm_batch_run": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc), ctypes.c_int,
+                                         ctypes.POINTER(ViewOut), ctypes.POINTER(BatchOpts)]),
+    "hp3d_pm_autotune": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc), ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int), _f32p]),
+    "hp3d_pm_variant_info": (ctypes.c_int, [ctypes.c_int, ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int),
+                                            ctypes.POINTER(ctypes.c_int)]),
+    "hp3d_pm_tune_save": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "hp3d_pm_tune_load": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_char_p]),
+    "hp3d_pm_geom_filter": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GeomView), ctypes.c_int,
+                                           ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                           ctypes.POINTER(ctypes.c_int)]),
+    "hp3d_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOpts),
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "hp3d_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(JobDesc),
+                                          ctypes.POINTER(ctypes.c_uint64)]),
+    "hp3d_server_poll": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64,
+                                        ctypes.POINTER(JobStatus)]),
+    "hp3d_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double,
+                                        ctypes.POINTER(JobStatus)]),
+    "hp3d_server_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "hp3d_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
+    "hp3d_server_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "hp3d_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "hp3d_pm_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_void_p)]),
     "hp3d_pm_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
@@ -305,6 +376,80 @@ class Context:
                       ctypes.cast(cost_ptr or None, _f32p), None, 1)
         _check(lib().hp3d_pm_run_view(self.h, ctypes.byref(desc), ctypes.byref(out)), self.h)
 
+    def batch_run(self, descs, shapes, variant=-1, use_graph=True, with_normal=True,
+                  with_cost=True):
+        """Solve many views in one graph launch; returns one output dict per view."""
+        n = len(descs)
+        arr = (ViewDesc * n)(*descs)
+        outs = (ViewOut * n)()
+        res = []
+        for j, (H, W) in enumerate(shapes):
+            r = {"depth": np.zeros((H, W), np.float32),
+                 "normal": np.zeros((H, W, 3), np.float32) if with_normal else None,
+                 "cost": np.zeros((H, W), np.float32) if with_cost else None}
+            outs[j] = ViewOut(_p(r["depth"], _f32p),
+                              _p(r["normal"], _f32p) if with_normal else None,
+                              _p(r["cost"], _f32p) if with_cost else None, None, 0)
+            res.append(r)
+        o = BatchOpts(int(variant), 1 if use_graph else 0)
+        _check(lib().hp3d_pm_batch_run(self.h, arr, n, outs, ctypes.byref(o)), self.h)
+        return res
+
+    def batch_run_device(self, descs, out_ptrs, variant=-1, use_graph=True):
+        """Batch into caller device buffers: out_ptrs = [(depth, normal, cost)] per view."""
+        n = len(descs)
+        arr = (ViewDesc * n)(*descs)
+        outs = (ViewOut * n)()
+        for j, (dd, dn, dc) in enumerate(out_ptrs):
+            outs[j] = ViewOut(ctypes.cast(dd, _f32p), ctypes.cast(dn or None, _f32p),
+                              ctypes.cast(dc or None, _f32p), None, 1)
+        o = BatchOpts(int(variant), 1 if use_graph else 0)
+        _check(lib().hp3d_pm_batch_run(self.h, arr, n, outs, ctypes.byref(o)), self.h)
+
+    def geom_filter(self, maps, depth_tol=0.12, normal_tol_deg=180.0, min_consistent=2):
+        """In-order, in-place geometric filter.  maps: list of dicts with keys
+        depth (H,W) float32 (modified), normal (H,W,3), confidence (optional,
+        modified), K, R, t, src (list of indices).  Returns retained counts."""
+        n = len(maps)
+        arr = (GeomView * n)()
+        for i, m in enumerate(maps):
+            for k in ("depth", "normal", "confidence"):
+                if m.get(k) is not None:
+                    assert m[k].dtype == np.float32 and m[k].flags.c_contiguous
+            g = arr[i]
+            g.height, g.width = m["depth"].shape
+            g.depth = _p(m["depth"], _f32p)
+            g.normal = _p(m["normal"], _f32p)
+            g.confidence = _p(m["confidence"], _f32p) if m.get("confidence") is not None else None
+            for j, val in enumerate(np.asarray(m["K"], np.float64).reshape(9)):
+                g.K[j] = val
+            for j, val in enumerate(np.asarray(m["R"], np.float64).reshape(9)):
+                g.R[j] = val
+            for j, val in enumerate(np.asarray(m["t"], np.float64).reshape(3)):
+                g.t[j] = val
+            g.num_src = len(m["src"])
+            for j, s in enumerate(m["src"]):
+                g.src[j] = int(s)
+        ret = (ctypes.c_int * n)()
+        _check(lib().hp3d_pm_geom_filter(self.h, arr, n, float(depth_tol), float(normal_tol_deg),
+                                         int(min_consistent), ret), self.h)
+        return list(ret)
+
+    def autotune(self, descs):
+        n = len(descs)
+        arr = (ViewDesc * n)(*descs)
+        best = ctypes.c_int(-1)
+        times = np.zeros(NUM_VARIANTS, np.float32)
+        _check(lib().hp3d_pm_autotune(self.h, arr, n, ctypes.byref(best), _p(times, _f32p)),
+               self.h)
+        return best.value, times
+
+    def tune_save(self, path):
+        _check(lib().hp3d_pm_tune_save(self.h, os.fsencode(path)), self.h)
+
+    def tune_load(self, path):
+        _check(lib().hp3d_pm_tune_load(self.h, os.fsencode(path)), self.h)
+
     def init_hypotheses(self, desc, shape):
         H, W = shape
         depth = np.zeros((H, W), dtype=np.float32)
@@ -354,3 +499,101 @@ def make_desc(ref, srcs, K_ref_inv, A, b, depth_min, depth_max, radius=5, stride
         d.init_depth = _p(idp, _f32p)
         d.init_normal = _p(inn, _f32p)
     return d
+
+
+# ---------------------------------------------------------------------------
+# Serving form
+# ---------------------------------------------------------------------------
+class Server:
+    """Continuous-batching request loop (hp3d_server_*): submit jobs at any time."""
+
+    def __init__(self, hbm_budget, max_jobs=32, radius=5, stride=1, store=STORE_F16, device=0,
+                 variant=-1):
+        o = ServerOpts(device, int(hbm_budget), max_jobs, radius, stride, store, variant)
+        h = ctypes.c_void_p()
+        code = lib().hp3d_server_create(ctypes.byref(o), ctypes.byref(h))
+        if code != OK:
+            raise HP3DError(code, "hp3d_server_create failed")
+        self.h = h
+        self._jobs = {}
+
+    def close(self):
+        if self.h:
+            lib().hp3d_server_destroy(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def submit(self, ref, srcs, K_ref_inv, A, b, depth_min, depth_max, iterations, seed=42,
+               top_k=3, with_normal=False, with_cost=True, fail_at_step=0):
+        """Queue one job; returns its id.  Output arrays are filled on completion."""
+        ref = np.ascontiguousarray(ref, dtype=np.uint8)
+        srcs = [np.ascontiguousarray(s, dtype=np.uint8) for s in srcs]
+        H, W = ref.shape
+        d = JobDesc()
+        d.width, d.height, d.num_src = W, H, len(srcs)
+        d.ref = _p(ref, _u8p)
+        for i, s in enumerate(srcs):
+            d.src[i] = _p(s, _u8p)
+        for i, val in enumerate(np.asarray(K_ref_inv, dtype=np.float64).reshape(9)):
+            d.K_ref_inv[i] = val
+        for s in range(len(srcs)):
+            for i, val in enumerate(np.asarray(A[s], dtype=np.float64).reshape(9)):
+                d.A[s][i] = val
+            for i, val in enumerate(np.asarray(b[s], dtype=np.float64).reshape(3)):
+                d.b[s][i] = val
+        d.depth_min, d.depth_max = float(depth_min), float(depth_max)
+        d.iterations, d.top_k, d.seed = int(iterations), int(top_k), int(seed)
+        out = {"depth": np.zeros((H, W), np.float32),
+               "normal": np.zeros((H, W, 3), np.float32) if with_normal else None,
+               "cost": np.zeros((H, W), np.float32) if with_cost else None}
+        d.out_depth = _p(out["depth"], _f32p)
+        d.out_normal = _p(out["normal"], _f32p) if with_normal else None
+        d.out_cost = _p(out["cost"], _f32p) if with_cost else None
+        d.fail_at_step = int(fail_at_step)
+        jid = ctypes.c_uint64(0)
+        code = lib().hp3d_server_submit(self.h, ctypes.byref(d), ctypes.byref(jid))
+        if code != OK:
+            raise HP3DError(code, lib().hp3d_server_last_error(self.h).decode())
+        self._jobs[jid.value] = out
+        return jid.value
+
+    def submit_job(self, job, **kw):
+        return self.submit(job.ref, job.srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                           kw.pop("iterations", job.iterations), seed=kw.pop("seed", job.seed),
+                           top_k=job.top_k, **kw)
+
+    @staticmethod
+    def _status(st):
+        return {f: getattr(st, f) for f, _ in JobStatus._fields_}
+
+    def poll(self, jid):
+        st = JobStatus()
+        _check(lib().hp3d_server_poll(self.h, jid, ctypes.byref(st)))
+        return self._status(st)
+
+    def wait(self, jid, timeout=-1.0):
+        st = JobStatus()
+        code = lib().hp3d_server_wait(self.h, jid, float(timeout), ctypes.byref(st))
+        if code == ERR_AGAIN:
+            return None
+        _check(code)
+        return self._status(st)
+
+    def result(self, jid):
+        return self._jobs[jid]
+
+    def forget(self, jid):
+        self._jobs.pop(jid, None)
+
+    def cancel(self, jid):
+        _check(lib().hp3d_server_cancel(self.h, jid))
+
+    def stats(self):
+        s = ServerStats()
+        _check(lib().hp3d_server_get_stats(self.h, ctypes.byref(s)))
+        return {f: getattr(s, f) for f, _ in ServerStats._fields_}

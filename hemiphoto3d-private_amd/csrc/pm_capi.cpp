@@ -129,6 +129,7 @@ void hp3d::fill_dev_view(DevView& v, const hp3d_view_desc* d, const Image* ref,
     v.ref_plain = ref->plain;
     v.pitch_plain = ref->pitch_plain;
     v.pitch_coef = ref->pitch_coef;
+    v.coef_bytes = (int)ref->coef_bytes;
     for (int i = 0; i < 9; ++i) {
         v.kinv[i] = (float)d->K_ref_inv[i];
         v.kinv_d[i] = d->K_ref_inv[i];
@@ -286,6 +287,9 @@ int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int W, int H, size_
     img.pitch_coef = (W + 63) & ~63;
     img.plain_bytes = (size_t)img.pitch_plain * H * esz;
     img.coef_bytes = (size_t)img.pitch_coef * H * esz * 4;
+    // taps address a coefficient plane with 32-bit buffer offsets
+    if (img.coef_bytes > (size_t)0x7fffffff)
+        return ctx->fail(HP3D_ERR_UNSUPPORTED, "image %dx%d: coefficient plane above 2 GiB", W, H);
     int rc = ctx->dev_alloc(&img.plain, img.plain_bytes, "image plain plane");
     if (rc) return rc;
     rc = ctx->dev_alloc(&img.coef, img.coef_bytes, "image coefficient plane");

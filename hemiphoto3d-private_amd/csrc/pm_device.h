@@ -40,6 +40,7 @@ struct DevView {
     const void* src_coef[kMaxSrc];      // source bilinear-coefficient planes
     int pitch_plain;                    // elements per row of a plain plane
     int pitch_coef;                     // elements (texels) per row of a coef plane
+    int coef_bytes;                     // bytes of one coef plane (buffer range check)
     float kinv[9];                      // K_ref^-1 (row-major), hypothesis arithmetic
     double kinv_d[9];                   // K_ref^-1 for the cost
     DevSrc src[kMaxSrc];

@@ -1,0 +1,442 @@
+// pm_pixel.h -- per-pixel device functions of the PatchMatch path, shared by
+// the per-view kernels (pm_kernels.hip) and the multi-view step kernel
+// (pm_step.hip).  Every function here is a pure function of its arguments and
+// the view's state, so a pixel computes the same bits in either kernel.
+//
+// Reference behaviour (src/dense_recon/patchmatch_mvs.cu):
+//   init_hyp        <- init_depth_normal_kernel      (:164-216)
+//   cand_cost       <- computePixelCost               (:222-343)
+//   cost_pixel      <- compute_ncc_cost_kernel body   (:349-513)
+//   halfpass_pixel  <- propagate_and_refine_kernel    (:519-655)
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pm_device.h"
+
+namespace hp3d {
+typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
+
+// ---------------------------------------------------------------------------
+// Philox4x32-10 and the u01 mapping ((x >> 8) + 1) * 2^-24 in (0, 1].
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void philox(uint32_t& c0, uint32_t& c1, uint32_t& c2, uint32_t& c3,
+                                       uint32_t k0, uint32_t k1)
+{
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        if (r > 0) { k0 += kPhiloxW0; k1 += kPhiloxW1; }
+        const uint32_t hi0 = __umulhi(kPhiloxM0, c0), lo0 = kPhiloxM0 * c0;
+        const uint32_t hi1 = __umulhi(kPhiloxM1, c2), lo1 = kPhiloxM1 * c2;
+        const uint32_t n0 = hi1 ^ c1 ^ k0, n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+    }
+}
+
+__device__ __forceinline__ float u01(uint32_t x)
+{
+    return (float)((x >> 8) + 1u) * (1.0f / 16777216.0f);
+}
+
+// Unit view ray K^-1 [x, y, 1] / |.|, written so it rounds like the host code.
+__device__ __forceinline__ void unit_ray(const float* ki, int x, int y, float& rx, float& ry,
+                                         float& rz)
+{
+    const float fx = (float)x, fy = (float)y;
+    float a = ki[0] * fx + ki[1] * fy + ki[2];
+    float b = ki[3] * fx + ki[4] * fy + ki[5];
+    float c = ki[6] * fx + ki[7] * fy + ki[8];
+    const float len = sqrtf(a * a + b * b + c * c);
+    if (len > 1e-12f) { a = a / len; b = b / len; c = c / len; }
+    rx = a; ry = b; rz = c;
+}
+
+// Unnormalised ray K^-1 [x, y, 1] in FP64 (the cost's back-projection, :241-243).
+__device__ __forceinline__ void ray_fp64(const double* ki, int x, int y, double& rx, double& ry,
+                                         double& rz)
+{
+    const double X = x, Y = y;
+    rx = ki[0] * X + ki[1] * Y + ki[2];
+    ry = ki[3] * X + ki[4] * Y + ki[5];
+    rz = ki[6] * X + ki[7] * Y + ki[8];
+}
+
+// ---------------------------------------------------------------------------
+// Image element access
+// ---------------------------------------------------------------------------
+template <int STORE> __device__ __forceinline__ float load_plain(const void* p, int off);
+template <> __device__ __forceinline__ float load_plain<kStoreF32>(const void* p, int off)
+{
+    return static_cast<const float*>(p)[off];
+}
+template <> __device__ __forceinline__ float load_plain<kStoreF16>(const void* p, int off)
+{
+    return (float)static_cast<const _Float16*>(p)[off];
+}
+
+// Bilinear-coefficient plane read through a raw buffer resource: the
+// hardware range-checks every access against the plane's size and returns 0
+// outside it, so a tap whose computed offset is wild (a lane that will be
+// masked) can never fault.  Dword 3 = 0x00020000 is the raw 32-bit data
+// format of gfx9-family buffer descriptors.
+constexpr int kBufferRsrcWord3 = 0x00020000;
+
+template <int STORE> struct CoefImage;
+template <> struct CoefImage<kStoreF32> {
+    __amdgpu_buffer_rsrc_t rsrc;
+    unsigned pitch;
+    __device__ CoefImage(const void* p, int bytes, int pc)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes,
+                                                 kBufferRsrcWord3)),
+          pitch((unsigned)pc) {}
+    __device__ __forceinline__ float4 load(int ix, int iy) const
+    {
+        const unsigned e = __umul24((unsigned)iy, pitch) + (unsigned)ix;
+        return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, e << 4, 0, 0));
+    }
+};
+template <> struct CoefImage<kStoreF16> {
+    __amdgpu_buffer_rsrc_t rsrc;
+    unsigned pitch;
+    __device__ CoefImage(const void* p, int bytes, int pc)
+        : rsrc(__builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes,
+                                                 kBufferRsrcWord3)),
+          pitch((unsigned)pc) {}
+    __device__ __forceinline__ float4 load(int ix, int iy) const
+    {
+        const unsigned e = __umul24((unsigned)iy, pitch) + (unsigned)ix;
+        const half4_t h =
+            __builtin_bit_cast(half4_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, e << 3, 0, 0));
+        return make_float4((float)h.x, (float)h.y, (float)h.z, (float)h.w);
+    }
+};
+
+// floor(x) converted to int in one instruction (v_cvt_flr_i32_f32; out of
+// range saturates, which the unsigned bounds test then rejects).
+__device__ __forceinline__ int cvt_flr(float x)
+{
+    int r;
+    asm("v_cvt_flr_i32_f32 %0, %1" : "=v"(r) : "v"(x));
+    return r;
+}
+
+struct TapSums {
+    float sr, ss, srr, sss, srs;
+    int cnt;
+};
+
+// The window's taps for one candidate and source (patchmatch_mvs.cu:273-307).
+// MASKED = false is used only when every tap is known to be inside the image;
+// both variants perform the same floating-point operations for valid taps, so
+// a window's sums do not depend on which variant its wave ran.
+template <int NT, int ST, int STORE, bool MASKED>
+__device__ __forceinline__ void tap_window(const CoefImage<STORE>& img,
+                                           const float* __restrict__ lwin, int lp, float cx,
+                                           float cy, float cz, float h00, float h01, float h10,
+                                           float h11, float h20, float h21, unsigned umax,
+                                           unsigned vmax, TapSums& m)
+{
+    constexpr int R = (NT - 1) * ST / 2;
+    float sr = 0.f, ss = 0.f, srr = 0.f, sss = 0.f, srs = 0.f;
+    int cnt = 0;
+#pragma unroll 1
+    for (int j = 0; j < NT; ++j) {
+        const float oy = (float)(j * ST - R);
+        const float bx = fmaf(h01, oy, cx);
+        const float by = fmaf(h11, oy, cy);
+        const float bz = fmaf(h21, oy, cz);
+        const float* lrow = lwin + j * ST * lp;
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            const float ox = (float)(i * ST - R);
+            const float wx = fmaf(h00, ox, bx);
+            const float wy = fmaf(h10, ox, by);
+            const float wz = fmaf(h20, ox, bz);
+            const float rw = __builtin_amdgcn_rcpf(wz);
+            const float u = wx * rw, w = wy * rw;
+            const int ix = cvt_flr(u), iy = cvt_flr(w);
+            const float4 c = img.load(ix, iy);
+            const float au = __builtin_amdgcn_fractf(u), aw = __builtin_amdgcn_fractf(w);
+            float sval = fmaf(aw, fmaf(au, c.w, c.z), fmaf(au, c.y, c.x));
+            float r = lrow[i * ST];
+            if (MASKED) {
+                // |w_z| guard and 0 <= u < W-1, 0 <= v < H-1 (:289-295)
+                const bool ok = (fabsf(wz) >= 1e-8f) & ((unsigned)ix <= umax) &
+                                ((unsigned)iy <= vmax);
+                sval = ok ? sval : 0.0f;
+                r = ok ? r : 0.0f;
+                cnt += ok ? 1 : 0;
+            }
+            sr += r;
+            ss += sval;
+            srr = fmaf(r, r, srr);
+            sss = fmaf(sval, sval, sss);
+            srs = fmaf(r, sval, srs);
+        }
+    }
+    m.sr = sr; m.ss = ss; m.srr = srr; m.sss = sss; m.srs = srs;
+    m.cnt = MASKED ? cnt : NT * NT;
+}
+
+// ---------------------------------------------------------------------------
+// Cost of one candidate plane (d, n) at pixel (x, y): patchmatch_mvs.cu:222-343.
+// lwin points at the pixel's window top-left tap in the LDS reference tile.
+// (rx, ry, rz) is the unnormalised ray K^-1 [x, y, 1] in FP64.
+//
+// Warp: w(p) = H p with H = A + b g^T, g = K^-T n / (n.P) (the reference's
+// q + b * factor, :280-292, in closed form).  The window centre w(x, y) and
+// the columns of H are formed in FP64; the taps are w(x, y) + dx h0 + dy h1
+// in FP32 with |dx|, |dy| <= 5, so the coordinates are accurate to the FP32
+// representation of w even for grazing planes.
+// ---------------------------------------------------------------------------
+template <int NT, int ST, int STORE>
+__device__ __forceinline__ float cand_cost(const DevView& v, const float* __restrict__ lwin,
+                                           int lp, int x, int y, double rx, double ry, double rz,
+                                           float d, float nx, float ny, float nz)
+{
+    constexpr int R = (NT - 1) * ST / 2;
+    constexpr int kMinCount = (NT * NT) / 2;
+    const double dd = d, dnx = nx, dny = ny, dnz = nz;
+    const double nP = dd * fma(dnx, rx, fma(dny, ry, dnz * rz));
+    if (fabs(nP) < 1e-8) return 2.0f;
+    const double inv = 1.0 / nP;
+    const double* ki = v.kinv_d;
+    // g = K^-T n / (n.P): the reference's factor(p) = g.p
+    const double g0 = fma(ki[0], dnx, fma(ki[3], dny, ki[6] * dnz)) * inv;
+    const double g1 = fma(ki[1], dnx, fma(ki[4], dny, ki[7] * dnz)) * inv;
+    const double g2 = fma(ki[2], dnx, fma(ki[5], dny, ki[8] * dnz)) * inv;
+    const double X = x, Y = y;
+    const double gc = fma(g0, X, fma(g1, Y, g2));   // factor at the window centre
+
+    float top[kMaxTopK];
+#pragma unroll
+    for (int k = 0; k < kMaxTopK; ++k) top[k] = 2.0f;
+
+    const unsigned umax = (unsigned)(v.W - 2), vmax = (unsigned)(v.H - 2);
+    const float Wm1 = (float)(v.W - 1), Hm1 = (float)(v.H - 1);
+
+    for (int s = 0; s < v.S; ++s) {
+        const double* A = v.src[s].A;
+        const double* b = v.src[s].b;
+        // centre of the window and the two tap-step columns of H
+        const float cx = (float)fma(b[0], gc, fma(A[0], X, fma(A[1], Y, A[2])));
+        const float cy = (float)fma(b[1], gc, fma(A[3], X, fma(A[4], Y, A[5])));
+        const float cz = (float)fma(b[2], gc, fma(A[6], X, fma(A[7], Y, A[8])));
+        const float h00 = (float)fma(b[0], g0, A[0]), h01 = (float)fma(b[0], g1, A[1]);
+        const float h10 = (float)fma(b[1], g0, A[3]), h11 = (float)fma(b[1], g1, A[4]);
+        const float h20 = (float)fma(b[2], g0, A[6]), h21 = (float)fma(b[2], g1, A[7]);
+        const CoefImage<STORE> img(v.src_coef[s], v.coef_bytes, v.pitch_coef);
+
+        // Every tap of this lane is inside the image if the four window
+        // corners are: w_z is affine in the tap offset, so one sign at the
+        // corners means no pole inside the window, and a projective map
+        // without a pole keeps the window inside its corners' convex hull.
+        // The 4e-3 px margin covers FP32 rounding of interior taps.
+        bool inside = true;
+        float sz0 = 0.0f;
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const float oy = (q & 2) ? (float)R : (float)-R, ox = (q & 1) ? (float)R : (float)-R;
+            const float wx = fmaf(h00, ox, fmaf(h01, oy, cx));
+            const float wy = fmaf(h10, ox, fmaf(h11, oy, cy));
+            const float wz = fmaf(h20, ox, fmaf(h21, oy, cz));
+            const float rw = __builtin_amdgcn_rcpf(wz);
+            const float u = wx * rw, w = wy * rw;
+            if (q == 0) sz0 = wz;
+            inside = inside & (fabsf(wz) >= 1e-7f) & ((wz > 0.0f) == (sz0 > 0.0f)) &
+                     (u >= 4e-3f) & (u <= Wm1 - 4e-3f) & (w >= 4e-3f) & (w <= Hm1 - 4e-3f);
+        }
+        TapSums m;
+        if (__all(inside))
+            tap_window<NT, ST, STORE, false>(img, lwin, lp, cx, cy, cz, h00, h01, h10, h11, h20,
+                                             h21, umax, vmax, m);
+        else
+            tap_window<NT, ST, STORE, true>(img, lwin, lp, cx, cy, cz, h00, h01, h10, h11, h20,
+                                            h21, umax, vmax, m);
+        const int cnt = m.cnt;
+        const float sr = m.sr, ss = m.ss, srr = m.srr, sss = m.sss, srs = m.srs;
+        float c = 2.0f;
+        if (cnt >= kMinCount) {
+            const double fc = (double)cnt;
+            const double num = fc * (double)srs - (double)sr * (double)ss;
+            const double dr = fc * (double)srr - (double)sr * (double)sr;
+            const double ds = fc * (double)sss - (double)ss * (double)ss;
+            if (!(dr < (double)v.den_eps) && !(ds < (double)v.den_eps)) {
+                float ncc = (float)num * rsqrtf((float)dr * (float)ds);
+                ncc = fmaxf(-1.0f, fminf(1.0f, ncc));
+                c = 1.0f - ncc;
+            }
+        }
+        // insert into the ascending list of the kMaxTopK smallest costs
+#pragma unroll
+        for (int k = kMaxTopK - 1; k > 0; --k)
+            top[k] = (c < top[k - 1]) ? top[k - 1] : (c < top[k] ? c : top[k]);
+        top[0] = c < top[0] ? c : top[0];
+    }
+    const int k = v.top_k < v.S ? v.top_k : v.S;
+    float sum = 0.0f;
+#pragma unroll
+    for (int i = 0; i < kMaxTopK; ++i)
+        if (i < k) sum += top[i];
+    return sum / (float)k;
+}
+
+// ---------------------------------------------------------------------------
+// Reference tile staging: rows y0-R .. y0+TY-1+R, columns 2*bx*TX-R ..
+// 2*bx*TX+2*TX-1+R, clamped to the image (texture clamp, :807-808).
+// ---------------------------------------------------------------------------
+template <int R, int STORE, int TX, int TY>
+struct TileGeom {
+    static constexpr int LW = 2 * TX + 2 * R;   // LDS tile width (pixels)
+    static constexpr int LH = TY + 2 * R;       // LDS tile height
+    static constexpr int LP = LW + 1;           // LDS row pitch
+};
+
+// Tile (bx, by) covers compacted columns bx*TX .. bx*TX+TX-1 (pixel columns
+// 2*bx*TX .. 2*bx*TX+2*TX-1) of rows by*TY .. by*TY+TY-1.
+template <int R, int STORE, int TX, int TY>
+__device__ __forceinline__ void stage_ref_tile(const DevView& v, float* tile, int bx, int by)
+{
+    using G = TileGeom<R, STORE, TX, TY>;
+    const int X0 = 2 * bx * TX - R;
+    const int Y0 = by * TY - R;
+    for (int k = threadIdx.x; k < G::LW * G::LH; k += TX * TY) {
+        const int ly = k / G::LW, lx = k - ly * G::LW;
+        const int gx = min(max(X0 + lx, 0), v.W - 1);
+        const int gy = min(max(Y0 + ly, 0), v.H - 1);
+        tile[ly * G::LP + lx] = load_plain<STORE>(v.ref_plain, gy * v.pitch_plain + gx);
+    }
+    __syncthreads();
+}
+
+// Random initial hypothesis of pixel (x, y): init_depth_normal_kernel (:164-216).
+__device__ __forceinline__ float4 init_hyp(const DevView& v, int x, int y, uint64_t seed)
+{
+    uint32_t c0 = (uint32_t)(y * v.W + x), c1 = 0u, c2 = 0u, c3 = kDomainInit;
+    philox(c0, c1, c2, c3, (uint32_t)seed, (uint32_t)(seed >> 32));
+    const float d = v.dmin + u01(c0) * (v.dmax - v.dmin);
+    float rx, ry, rz;
+    unit_ray(v.kinv, x, y, rx, ry, rz);
+    const float theta = u01(c1) * 6.28318530f;
+    const float cos_phi = u01(c2);
+    const float sin_phi = sqrtf(1.0f - cos_phi * cos_phi);
+    float nx = sin_phi * (float)cos((double)theta);
+    float ny = sin_phi * (float)sin((double)theta);
+    float nz = cos_phi;
+    if (nx * rx + ny * ry + nz * rz > 0.0f) { nx = -nx; ny = -ny; nz = -nz; }
+    const float nlen = sqrtf(nx * nx + ny * ny + nz * nz);
+    if (nlen > 1e-12f) { nx = nx / nlen; ny = ny / nlen; nz = nz / nlen; }
+    return make_float4(d, nx, ny, nz);
+}
+
+// Cost of the current hypothesis of the colour-`parity` pixel of lane (tx, ty)
+// in tile (bx, by); returns false if the lane has no pixel.
+template <int NT, int ST, int STORE, int TX, int TY>
+__device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, int bx, int by,
+                                           int tx, int ty, int parity)
+{
+    constexpr int R = (NT - 1) * ST / 2;
+    using G = TileGeom<R, STORE, TX, TY>;
+    const int i = bx * TX + tx;
+    const int y = by * TY + ty;
+    const int x = 2 * i + ((y + parity) & 1);
+    if (y >= v.H || x >= v.W) return false;
+    const float* lwin = tile + ty * G::LP + (x - 2 * bx * TX);
+    double rx, ry, rz;
+    ray_fp64(v.kinv_d, x, y, rx, ry, rz);
+    const int p = y * v.Wc + i;
+    const float4 h = v.hyp[parity][p];
+    v.cost[parity][p] =
+        cand_cost<NT, ST, STORE>(v, lwin, G::LP, x, y, rx, ry, rz, h.x, h.y, h.z, h.w);
+    return true;
+}
+
+// One pixel of a red (parity 0) or black (parity 1) half-pass:
+// propagate_and_refine_kernel (:519-655).  Candidates L, R, U, D from the
+// other colour plane, then two random refinements with ranges
+// (dmax-dmin)/2 * 0.6^iter and 0.6^iter, halved per step.  Strict '<'
+// acceptance throughout.
+template <int NT, int ST, int STORE, int TX, int TY>
+__device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* tile, int bx, int by,
+                                               int tx, int ty, int iteration, int parity,
+                                               uint64_t pass_seed)
+{
+    constexpr int R = (NT - 1) * ST / 2;
+    using G = TileGeom<R, STORE, TX, TY>;
+    const int i = bx * TX + tx;
+    const int y = by * TY + ty;
+    const int o = (y + parity) & 1;
+    const int x = 2 * i + o;
+    if (y >= v.H || x >= v.W) return;
+    const float* lwin = tile + ty * G::LP + (x - 2 * bx * TX);
+    double rx, ry, rz;
+    ray_fp64(v.kinv_d, x, y, rx, ry, rz);
+    const float dmin = v.dmin, dmax = v.dmax;
+    const int Wc = v.Wc;
+
+    float4* own = v.hyp[parity];
+    const float4* oth = v.hyp[parity ^ 1];
+    const int p = y * Wc + i;
+    float4 best = own[p];
+    float bc = v.cost[parity][p];
+
+    // neighbour positions in the other plane: L = i+o-1, R = i+o (row y),
+    // U, D = i (rows y-1, y+1)
+#pragma unroll 1
+    for (int k = 0; k < 4; ++k) {
+        const bool has = k == 0 ? x >= 1 : k == 1 ? x + 1 < v.W : k == 2 ? y >= 1 : y + 1 < v.H;
+        if (!has) continue;
+        const int q = k == 0 ? p + o - 1 : k == 1 ? p + o : k == 2 ? p - Wc : p + Wc;
+        const float4 h = oth[q];
+        if (h.x < dmin || h.x > dmax) continue;
+        const float c = cand_cost<NT, ST, STORE>(v, lwin, G::LP, x, y, rx, ry, rz, h.x, h.y, h.z,
+                                                 h.w);
+        if (c < bc) { bc = c; best = h; }
+    }
+
+    float ux, uy, uz;
+    unit_ray(v.kinv, x, y, ux, uy, uz);
+    float depth_range = (dmax - dmin) * 0.5f;
+    float normal_range = 1.0f;
+    for (int it = 0; it < iteration; ++it) { depth_range *= 0.6f; normal_range *= 0.6f; }
+    const uint32_t idx = (uint32_t)(y * v.W + x);
+#pragma unroll 1
+    for (int step = 0; step < 2; ++step) {
+        uint32_t c0 = idx, c1 = (uint32_t)iteration, c2 = (uint32_t)step, c3 = kDomainRefine;
+        philox(c0, c1, c2, c3, (uint32_t)pass_seed, (uint32_t)(pass_seed >> 32));
+        const float rand_d = u01(c0) * 2.0f - 1.0f;
+        float cd = best.x + rand_d * depth_range;
+        cd = fmaxf(dmin, fminf(dmax, cd));
+        float cnx = best.y + (u01(c1) * 2.0f - 1.0f) * normal_range;
+        float cny = best.z + (u01(c2) * 2.0f - 1.0f) * normal_range;
+        float cnz = best.w + (u01(c3) * 2.0f - 1.0f) * normal_range;
+        const float nlen = sqrtf(cnx * cnx + cny * cny + cnz * cnz);
+        if (nlen > 1e-12f) { cnx = cnx / nlen; cny = cny / nlen; cnz = cnz / nlen; }
+        if (cnx * ux + cny * uy + cnz * uz > 0.0f) { cnx = -cnx; cny = -cny; cnz = -cnz; }
+        const float c = cand_cost<NT, ST, STORE>(v, lwin, G::LP, x, y, rx, ry, rz, cd, cnx, cny,
+                                                 cnz);
+        if (c < bc) { bc = c; best = make_float4(cd, cnx, cny, cnz); }
+        depth_range *= 0.5f;
+        normal_range *= 0.5f;
+    }
+    own[p] = best;
+    v.cost[parity][p] = bc;
+}
+
+// Row-major outputs of pixel (x, y) from the colour planes.
+__device__ __forceinline__ void emit_pixel(const DevView& v, int x, int y, float* depth,
+                                           float* normal, float* cost, float* conf)
+{
+    const int c = (x + y) & 1, p = y * v.Wc + (x >> 1);
+    const size_t q = (size_t)y * v.W + x;
+    const float4 h = v.hyp[c][p];
+    if (depth) depth[q] = h.x;
+    if (normal) { normal[3 * q] = h.y; normal[3 * q + 1] = h.z; normal[3 * q + 2] = h.w; }
+    if (cost || conf) {
+        const float k = v.cost[c][p];
+        if (cost) cost[q] = k;
+        if (conf) conf[q] = 1.0f - k * 0.5f;   // patchmatch_mvs.cu:1249-1252
+    }
+}
+
+}  // namespace hp3d

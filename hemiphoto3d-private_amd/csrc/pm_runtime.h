@@ -2,6 +2,7 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <map>
 #include <string>
 #include <unordered_map>
 #include <vector>
@@ -50,6 +51,12 @@ struct hp3d_pm_ctx {
     // per-view working set, grown on demand and reused across views
     hp3d::DevBuf hyp, cost, out_depth, out_normal, out_cost, in_depth, in_normal, staging;
 
+    // batched form: state arena, job table, step items, captured graph
+    hp3d::DevBuf batch_state, batch_jobs, batch_items;
+    hipGraphExec_t batch_graph = nullptr;
+    std::vector<unsigned char> batch_graph_key;
+    std::map<int, int> tuned;   // (radius, stride, store) key -> variant
+
     hp3d_pm_stats stats{};
     std::vector<hp3d::Mark> marks;
 
@@ -77,8 +84,12 @@ struct hp3d_pm_ctx {
         images.clear();
         for (auto& kv : user_bufs) dev_free(kv.first, kv.second);
         user_bufs.clear();
+        if (batch_graph) {
+            (void)hipGraphExecDestroy(batch_graph);
+            batch_graph = nullptr;
+        }
         for (hp3d::DevBuf* b : {&hyp, &cost, &out_depth, &out_normal, &out_cost, &in_depth,
-                                &in_normal, &staging})
+                                &in_normal, &staging, &batch_state, &batch_jobs, &batch_items})
             release_buf(*b);
     }
 };

@@ -1,0 +1,143 @@
+// pm_step.hip -- multi-view step kernel: one launch advances every active
+// view of a batch or of the serving loop by one pass.
+//
+// A step is a list of items, one per active view (job): START (random init
+// and initial cost of both colours, patchmatch_mvs.cu:1107-1196) or PROPAGATE
+// (one red/black half-pass, :1208-1226), optionally followed by writing the
+// view's outputs (:1243-1252).  Each view advances at its own pass index, so
+// views of any size, source count and iteration count share a launch and new
+// views can join between steps.
+//
+// Work decomposition: every view is cut into TX x TY compacted tiles; all
+// items' tiles form one logical range.  Workgroups are dispatched to the 8
+// XCDs round-robin (block b -> XCD b % 8), so block b takes logical tile
+// (b % 8) * (nblocks / 8) + b / 8: each XCD works through a contiguous run of
+// tiles, and tiles that read the same source rows share that XCD's L2.
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pm_device.h"
+#include "pm_pixel.h"
+#include "pm_step.h"
+
+namespace hp3d {
+
+template <int NT, int ST, int STORE, int TX, int TY, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+void pm_step_kernel(const DevJob* __restrict__ jobs, const StepItem* __restrict__ items,
+                    int n_items, int total_tiles)
+{
+    constexpr int R = (NT - 1) * ST / 2;
+    using G = TileGeom<R, STORE, TX, TY>;
+    __shared__ float tile[G::LH * G::LP];
+
+    const int nb = gridDim.x;                       // a multiple of 8
+    const int b = blockIdx.x;
+    const int t = (b & 7) * (nb >> 3) + (b >> 3);
+    if (t >= total_tiles) return;
+    int lo = 0, hi = n_items - 1;                   // last item with tile_begin <= t
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (items[mid].tile_begin <= t) lo = mid;
+        else hi = mid - 1;
+    }
+    const StepItem& it = items[lo];
+    const DevJob& job = jobs[it.job];
+    const DevView& v = job.v;
+    const int lt = t - it.tile_begin;
+    const int by = lt / it.tiles_x, bx = lt - by * it.tiles_x;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+
+    stage_ref_tile<R, STORE, TX, TY>(v, tile, bx, by);
+    const int i = bx * TX + tx, y = by * TY + ty;
+    if (it.mode == kStepStart) {
+        if (y < v.H) {
+            for (int c = 0; c < 2; ++c) {
+                const int x = 2 * i + ((y + c) & 1);
+                if (x < v.W) v.hyp[c][y * v.Wc + i] = init_hyp(v, x, y, job.seed);
+            }
+        }
+        cost_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, tx, ty, 0);
+        cost_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, tx, ty, 1);
+    } else {
+        halfpass_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, tx, ty, it.iteration, it.parity,
+                                              it.pass_seed);
+    }
+    // outputs: this lane's pixel of each colour; the other colour was
+    // completed by the previous step
+    if (it.emit && y < v.H) {
+        for (int c = 0; c < 2; ++c) {
+            const int x = 2 * i + ((y + c) & 1);
+            if (x < v.W) emit_pixel(v, x, y, job.out_depth, job.out_normal, job.out_cost,
+                                    job.out_conf);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Kernel variants.  Every window has the default variant; the windows of the
+// batched and serving workloads (radius 5, stride 1 or 2) also have the tile
+// shapes and occupancies the autotuner chooses from.
+// ---------------------------------------------------------------------------
+template <int NT, int ST, int STORE, int TX, int TY, int WPE>
+static hipError_t launch_t(const DevJob* jobs, const StepItem* items, int n_items, int total_tiles,
+                           hipStream_t st)
+{
+    const int grid = ((total_tiles + 7) / 8) * 8;
+    hipLaunchKernelGGL((pm_step_kernel<NT, ST, STORE, TX, TY, WPE>), dim3(grid), dim3(TX * TY), 0,
+                       st, jobs, items, n_items, total_tiles);
+    return hipGetLastError();
+}
+
+const StepVariant kStepVariants[] = {
+    {16, 16, 1}, {32, 8, 1}, {8, 32, 1}, {64, 4, 1}, {16, 16, 5}, {32, 8, 5},
+};
+const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
+
+template <int NT, int ST, int STORE>
+static hipError_t launch_tuned(int variant, const DevJob* jobs, const StepItem* items, int n_items,
+                               int total_tiles, hipStream_t st)
+{
+    switch (variant) {
+        case 0: return launch_t<NT, ST, STORE, 16, 16, 1>(jobs, items, n_items, total_tiles, st);
+        case 1: return launch_t<NT, ST, STORE, 32, 8, 1>(jobs, items, n_items, total_tiles, st);
+        case 2: return launch_t<NT, ST, STORE, 8, 32, 1>(jobs, items, n_items, total_tiles, st);
+        case 3: return launch_t<NT, ST, STORE, 64, 4, 1>(jobs, items, n_items, total_tiles, st);
+        case 4: return launch_t<NT, ST, STORE, 16, 16, 5>(jobs, items, n_items, total_tiles, st);
+        case 5: return launch_t<NT, ST, STORE, 32, 8, 5>(jobs, items, n_items, total_tiles, st);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+bool step_variant_tunable(int radius, int stride) { return radius == 5; }
+
+int step_tile_tx(int variant) { return kStepVariants[variant].tx; }
+int step_tile_ty(int variant) { return kStepVariants[variant].ty; }
+
+hipError_t launch_step(int store, int radius, int stride, int variant, const DevJob* jobs,
+                       const StepItem* items, int n_items, int total_tiles, hipStream_t st)
+{
+    if (variant < 0 || variant >= kNumStepVariants) return hipErrorInvalidValue;
+    if (!step_variant_tunable(radius, stride) && variant != 0) return hipErrorInvalidValue;
+#define HP3D_TUNED(R_, S_, ST_)                                                                  \
+    if (radius == R_ && stride == S_ && store == ST_)                                           \
+        return launch_tuned<(2 * R_) / S_ + 1, S_, ST_>(variant, jobs, items, n_items, total_tiles, \
+                                                        st);
+#define HP3D_PLAIN(R_, S_, ST_)                                                                 \
+    if (radius == R_ && stride == S_ && store == ST_)                                          \
+        return launch_t<(2 * R_) / S_ + 1, S_, ST_, 16, 16, 1>(jobs, items, n_items, total_tiles, \
+                                                                st);
+    HP3D_TUNED(5, 1, kStoreF32) HP3D_TUNED(5, 2, kStoreF32)
+    HP3D_TUNED(5, 1, kStoreF16) HP3D_TUNED(5, 2, kStoreF16)
+    HP3D_PLAIN(1, 1, kStoreF32) HP3D_PLAIN(2, 1, kStoreF32) HP3D_PLAIN(3, 1, kStoreF32)
+    HP3D_PLAIN(4, 1, kStoreF32) HP3D_PLAIN(1, 2, kStoreF32) HP3D_PLAIN(2, 2, kStoreF32)
+    HP3D_PLAIN(3, 2, kStoreF32) HP3D_PLAIN(4, 2, kStoreF32)
+    HP3D_PLAIN(1, 1, kStoreF16) HP3D_PLAIN(2, 1, kStoreF16) HP3D_PLAIN(3, 1, kStoreF16)
+    HP3D_PLAIN(4, 1, kStoreF16) HP3D_PLAIN(1, 2, kStoreF16) HP3D_PLAIN(2, 2, kStoreF16)
+    HP3D_PLAIN(3, 2, kStoreF16) HP3D_PLAIN(4, 2, kStoreF16)
+#undef HP3D_TUNED
+#undef HP3D_PLAIN
+    return hipErrorInvalidValue;
+}
+
+}  // namespace hp3d

@@ -145,6 +145,138 @@ int hp3d_pm_cost_eval(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, const float*
 
 int hp3d_pm_get_stats(const hp3d_pm_ctx* ctx, hp3d_pm_stats* out);
 
+/* ---------------------------------------------------------------------------
+ * Geometric consistency filter over a set of depth maps (replaces
+ * geometric_consistency_filter_kernel and its driver, patchmatch_mvs.cu:668-781
+ * and :1267-1384).  Views are filtered in array order, in place: a pixel keeps
+ * its depth if at least min_consistent sources see it within depth_tol
+ * relative depth (and normal_tol_deg, 180 = off); later views see earlier
+ * views already filtered, as in the reference's host loop.  Sources of a
+ * different size are skipped; a view with fewer usable sources than
+ * min_consistent is left as is.  The reference runs it with 0.12, 180, 2
+ * (:1273-1275).  Host memory in and out; the maps stay on the device for
+ * the whole set.  retained (nullable): per view, pixels left (-1: unfiltered).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int width, height;
+    float* depth;           /* H*W, filtered in place (0 = invalid) */
+    const float* normal;    /* H*W*3, camera frame */
+    float* confidence;      /* H*W, zeroed with depth (nullable) */
+    double K[9], R[9], t[3];
+    int num_src;
+    int src[HP3D_PM_MAX_SRC];   /* indices into the view array */
+} hp3d_geom_view;
+
+int hp3d_pm_geom_filter(hp3d_pm_ctx* ctx, hp3d_geom_view* views, int n, float depth_tol,
+                        float normal_tol_deg, int min_consistent, int* retained);
+
+/* ---------------------------------------------------------------------------
+ * Batched form: many reference views of any size, source count and
+ * iteration count solved together.  The reference solves views one by one
+ * in a host loop (patchmatch_mvs.cu:1076-1258) with a device synchronisation
+ * after every launch; here one step kernel advances every view by one pass
+ * and the whole step sequence is captured in one hipGraph and launched once.
+ * All views of a batch share radius, stride and image storage.  Results are
+ * bit-identical to hp3d_pm_run_view on the same view.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int variant;     /* step-kernel variant; -1 = the autotuned choice (or the default) */
+    int use_graph;   /* 1 = capture the step sequence in one hipGraph (default) */
+} hp3d_batch_opts;
+
+int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
+                      const hp3d_view_out* outs, const hp3d_batch_opts* opts);
+
+/* Autotuner: times every step-kernel variant (tile shape x waves per SIMD)
+ * on the given views for one start step and two half-passes, on scratch
+ * state, and records the fastest for their (radius, stride, storage).
+ * times_ms (nullable) receives HP3D_PM_NUM_VARIANTS entries, -1 where a
+ * variant does not apply. */
+#define HP3D_PM_NUM_VARIANTS 6
+int hp3d_pm_autotune(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n, int* best_variant,
+                     float* times_ms);
+/* Tile shape and waves-per-SIMD hint of a variant. */
+int hp3d_pm_variant_info(int variant, int* tile_cols, int* tile_rows, int* waves_per_simd);
+/* Persist / restore the autotuner's choices (text, one line per key). */
+int hp3d_pm_tune_save(const hp3d_pm_ctx* ctx, const char* path);
+int hp3d_pm_tune_load(hp3d_pm_ctx* ctx, const char* path);
+
+/* ---------------------------------------------------------------------------
+ * Serving form: a request loop with continuous batching.  Jobs (one
+ * reference view each, with its own images) are submitted at any time; a
+ * scheduler thread admits them into free slots when their device footprint
+ * (images + state + outputs) fits the server's HBM budget, and launches one
+ * step at a time over every admitted job, each at its own pass.  Jobs join
+ * and leave between steps; images, hypothesis state and outputs stay on the
+ * device until the job completes.  A job's result is bit-identical to
+ * hp3d_pm_run_view on the same view, whatever else runs beside it.
+ * There is no counterpart in the reference, whose MVS runs only inside the
+ * blocking runPatchMatchMVS call (patchmatch_mvs.cu:1028).
+ * ------------------------------------------------------------------------- */
+typedef struct hp3d_pm_server hp3d_pm_server;
+
+typedef struct {
+    int device;
+    size_t hbm_budget;    /* device arena reserved for admitted jobs (bytes) */
+    int max_jobs;         /* concurrent job slots */
+    int radius, stride;   /* window shared by every job */
+    int store;            /* hp3d_storage of every job's images */
+    int variant;          /* step-kernel variant, -1 = default */
+} hp3d_server_opts;
+
+typedef struct {
+    int width, height, num_src;
+    const uint8_t* ref;                      /* host 8-bit images, copied at submit */
+    const uint8_t* src[HP3D_PM_MAX_SRC];
+    double K_ref_inv[9];
+    double A[HP3D_PM_MAX_SRC][9];
+    double b[HP3D_PM_MAX_SRC][3];
+    float depth_min, depth_max;
+    int iterations, top_k;
+    uint64_t seed;
+    float* out_depth;                        /* host, H*W: written when the job completes */
+    float* out_normal;                       /* host, H*W*3, nullable */
+    float* out_cost;                         /* host, H*W, nullable */
+    int fail_at_step;                        /* test hook: > 0 fails the job at that step */
+} hp3d_job_desc;
+
+typedef enum {
+    HP3D_JOB_QUEUED = 0, HP3D_JOB_RUNNING = 1, HP3D_JOB_DONE = 2,
+    HP3D_JOB_FAILED = 3, HP3D_JOB_CANCELLED = 4
+} hp3d_job_state;
+
+typedef struct {
+    int state;             /* hp3d_job_state */
+    int error;             /* hp3d_status of a failed job */
+    double t_submit;       /* seconds on the server's steady clock */
+    double t_admit;
+    double t_done;
+    int steps;             /* steps the job took part in */
+} hp3d_job_status;
+
+typedef struct {
+    long long steps;            /* step launches */
+    double busy_ms;             /* device time of all steps */
+    double wall_s;              /* since the server started */
+    long long submitted, done, failed, cancelled, rejected;
+    double taps;                /* algorithmic taps of completed jobs */
+    double mpix_iters;          /* W*H*iterations of completed jobs / 1e6 */
+    size_t arena_bytes, arena_peak;
+    int max_active;             /* most jobs in one step */
+} hp3d_server_stats;
+
+int hp3d_server_create(const hp3d_server_opts* opts, hp3d_pm_server** out);
+/* Queue a job; HP3D_ERR_OOM_BUDGET if it could never fit the budget. */
+int hp3d_server_submit(hp3d_pm_server* srv, const hp3d_job_desc* job, uint64_t* id);
+int hp3d_server_poll(hp3d_pm_server* srv, uint64_t id, hp3d_job_status* st);
+/* Block until the job leaves the queue and the device (timeout_s < 0: forever);
+ * HP3D_ERR_AGAIN on timeout. */
+int hp3d_server_wait(hp3d_pm_server* srv, uint64_t id, double timeout_s, hp3d_job_status* st);
+int hp3d_server_cancel(hp3d_pm_server* srv, uint64_t id);
+int hp3d_server_get_stats(hp3d_pm_server* srv, hp3d_server_stats* st);
+const char* hp3d_server_last_error(hp3d_pm_server* srv);
+int hp3d_server_destroy(hp3d_pm_server* srv);
+
 /* Device buffers owned by the context and counted against its HBM budget,
  * for callers without a HIP runtime of their own (the reference keeps its
  * device state in DeviceBuffer<T>, src/utils/cuda_utils.cuh:26-57).

@@ -1,0 +1,122 @@
+"""GPU tests of the serving form (hp3d_server_*): continuous batching.
+
+Each job's result must equal its per-view solve bit for bit, alone and in a
+mix of jobs that arrive while others run; admission respects the HBM budget;
+a cancelled or failed job leaves the others untouched.
+"""
+import time
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+MB = 1 << 20
+
+
+@pytest.fixture(scope="module")
+def serve_jobs(scenes):
+    specs = [(200, 150, 3, 4), (128, 96, 5, 2), (256, 192, 4, 6), (96, 72, 8, 3), (160, 240, 6, 5),
+             (180, 120, 3, 1)]
+    return [scenes.sphere_job(W, H, S, scene_seed=40 + k, radius=5, stride=2, iterations=it,
+                              seed=7 + 1000000 * k, name=f"srv{k}")
+            for k, (W, H, S, it) in enumerate(specs)]
+
+
+@pytest.fixture(scope="module")
+def expected(pkg, ctx, serve_jobs):
+    out = []
+    for job in serve_jobs:
+        ref = ctx.upload(job.ref)
+        srcs = [ctx.upload(s) for s in job.srcs]
+        d = pkg.make_desc(ref, srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax, radius=5,
+                          stride=2, iterations=job.iterations, seed=job.seed)
+        out.append(ctx.run_view(d, job.shape))
+    return out
+
+
+def _check(res, exp):
+    assert np.array_equal(res["depth"], exp["depth"])
+    assert np.array_equal(res["cost"], exp["cost"])
+
+
+def test_single_job_matches_run_view(pkg, serve_jobs, expected):
+    with pkg.Server(256 * MB, max_jobs=4, stride=2) as srv:
+        jid = srv.submit_job(serve_jobs[0])
+        st = srv.wait(jid, timeout=60)
+        assert st["state"] == pkg.JOB_DONE
+        assert st["steps"] == 1 + 2 * serve_jobs[0].iterations
+        _check(srv.result(jid), expected[0])
+
+
+def test_mix_matches_alone(pkg, serve_jobs, expected):
+    with pkg.Server(512 * MB, max_jobs=8, stride=2) as srv:
+        ids = [srv.submit_job(j) for j in serve_jobs]
+        for k, jid in enumerate(ids):
+            st = srv.wait(jid, timeout=120)
+            assert st["state"] == pkg.JOB_DONE, st
+            _check(srv.result(jid), expected[k])
+        s = srv.stats()
+        assert s["done"] == len(serve_jobs) and s["max_active"] >= 2
+        print("serve stats:", s)
+
+
+def test_staggered_arrivals_and_few_slots(pkg, serve_jobs, expected):
+    with pkg.Server(512 * MB, max_jobs=2, stride=2) as srv:
+        ids = []
+        for j in serve_jobs:
+            ids.append(srv.submit_job(j))
+            time.sleep(0.003)
+        for k, jid in enumerate(ids):
+            assert srv.wait(jid, timeout=120)["state"] == pkg.JOB_DONE
+            _check(srv.result(jid), expected[k])
+        assert srv.stats()["max_active"] <= 2
+
+
+def test_budget_admission(pkg, serve_jobs, expected):
+    small = 40 * MB
+    with pkg.Server(small, max_jobs=8, stride=2) as srv:
+        big = serve_jobs[2]
+        ids = [srv.submit_job(j) for j in serve_jobs]
+        for k, jid in enumerate(ids):
+            assert srv.wait(jid, timeout=120)["state"] == pkg.JOB_DONE
+            _check(srv.result(jid), expected[k])
+        s = srv.stats()
+        assert s["arena_peak"] <= small
+        # a job that can never fit is refused at submit
+        huge = np.zeros((3000, 4000), np.uint8)
+        with pytest.raises(pkg.HP3DError) as e:
+            srv.submit(huge, [huge, huge], big.Kinv, big.A[:2], big.b[:2], 2.0, 12.0, 1)
+        assert e.value.code == pkg.ERR_OOM_BUDGET
+        assert srv.stats()["rejected"] == 1
+
+
+def test_cancel_and_injected_failure_leave_others_intact(pkg, serve_jobs, expected):
+    with pkg.Server(512 * MB, max_jobs=1, stride=2) as srv:
+        first = srv.submit_job(serve_jobs[2])                    # occupies the only slot
+        failing = srv.submit_job(serve_jobs[1], fail_at_step=2)
+        cancelled = srv.submit_job(serve_jobs[3])
+        srv.cancel(cancelled)
+        last = srv.submit_job(serve_jobs[0])
+        assert srv.wait(first, 120)["state"] == pkg.JOB_DONE
+        st_f = srv.wait(failing, 120)
+        assert st_f["state"] == pkg.JOB_FAILED and st_f["steps"] == 2
+        assert srv.wait(cancelled, 120)["state"] == pkg.JOB_CANCELLED
+        assert srv.wait(last, 120)["state"] == pkg.JOB_DONE
+        _check(srv.result(first), expected[2])
+        _check(srv.result(last), expected[0])
+        s = srv.stats()
+        assert (s["done"], s["failed"], s["cancelled"]) == (2, 1, 1)
+
+
+def test_invalid_submissions(pkg, serve_jobs):
+    with pkg.Server(64 * MB, max_jobs=2, stride=2) as srv:
+        j = serve_jobs[0]
+        with pytest.raises(pkg.HP3DError) as e:
+            srv.submit(j.ref, [], j.Kinv, [], [], 2.0, 12.0, 1)
+        assert e.value.code == pkg.ERR_INVALID_ARG
+        with pytest.raises(pkg.HP3DError) as e:
+            srv.submit(j.ref, j.srcs, j.Kinv, j.A, j.b, 5.0, 5.0, 1)
+        assert e.value.code == pkg.ERR_INVALID_ARG
+        with pytest.raises(pkg.HP3DError):
+            srv.poll(123456)
