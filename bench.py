@@ -43,15 +43,36 @@ def parse():
     ap.add_argument("--cpu-threads", type=int, default=min(16, os.cpu_count() or 1))
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--scale", type=float, default=1.0, help="shrink W, H (not for reporting)")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run one device-to-device copy of MIB MiB (counter calibration)")
     return ap.parse_args()
+
+
+def copy_probe(mib):
+    """One plain device-to-device copy of known size: under rocprofv3 --pmc its
+    FETCH_SIZE / WRITE_SIZE per dispatch calibrate the traffic counters."""
+    import __graft_entry__ as entry
+    pkg = entry.load_package()
+    ctx = pkg.Context()
+    n = mib << 20
+    src, dst = ctx.device_alloc(n), ctx.device_alloc(n)
+    ctx.upload_bytes(src, np.zeros(n, np.uint8))
+    import ctypes
+    rc = pkg.lib().hp3d_pm_device_copy(ctx.h, ctypes.c_void_p(dst), ctypes.c_void_p(src), n, 2)
+    assert rc == 0, ctx.last_error()
+    ctx.close()
+    print(json.dumps({"copy_probe_bytes": n}))
 
 
 def main():
     a = parse()
     if a.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print("bench.py: this path runs on one GPU; multi-GPU sharding is a design note only "
-              "(DESIGN.md), so --gpus must be 1", file=sys.stderr)
+        print("bench.py: this path runs on one GPU; multi-GPU sharding is a design note only, "
+              "so --gpus must be 1", file=sys.stderr)
         sys.exit(2)
+    if a.copy_probe:
+        copy_probe(a.copy_probe)
+        return
     import __graft_entry__ as entry
     from importlib import import_module
     pkg = entry.load_package()

@@ -128,13 +128,14 @@ void hp3d::fill_dev_view(DevView& v, const hp3d_view_desc* d, const Image* ref,
     std::memset(&v, 0, sizeof v);
     v.ref_plain = ref->plain;
     v.pitch_plain = ref->pitch_plain;
-    v.pitch_coef = ref->pitch_coef;
+    v.plain_bytes = (int)ref->plain_bytes;
     v.coef_bytes = (int)ref->coef_bytes;
     for (int i = 0; i < 9; ++i) {
         v.kinv[i] = (float)d->K_ref_inv[i];
         v.kinv_d[i] = d->K_ref_inv[i];
     }
     for (int s = 0; s < d->num_src; ++s) {
+        v.src_plain[s] = srcs[s]->plain;
         v.src_coef[s] = srcs[s]->coef;
         for (int i = 0; i < 9; ++i) v.src[s].A[i] = d->A[s][i];
         for (int i = 0; i < 3; ++i) v.src[s].b[i] = d->b[s][i];
@@ -282,35 +283,33 @@ int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int W, int H, size_
     img.H = H;
     img.store = store;
     img.den_scale = 255.0f * 255.0f;
-    const size_t esz = store == HP3D_STORE_F32 ? 4 : 2;
-    img.pitch_plain = (W + 63) & ~63;
-    img.pitch_coef = (W + 63) & ~63;
-    img.plain_bytes = (size_t)img.pitch_plain * H * esz;
-    img.coef_bytes = (size_t)img.pitch_coef * H * esz * 4;
-    // taps address a coefficient plane with 32-bit buffer offsets
-    if (img.coef_bytes > (size_t)0x7fffffff)
-        return ctx->fail(HP3D_ERR_UNSUPPORTED, "image %dx%d: coefficient plane above 2 GiB", W, H);
-    int rc = ctx->dev_alloc(&img.plain, img.plain_bytes, "image plain plane");
+    image_plane_sizes(W, H, store, &img.pitch_plain, &img.plain_bytes, &img.coef_bytes);
+    // taps address a plane with 32-bit buffer offsets
+    if (img.plain_bytes > (size_t)0x7fffffff || img.coef_bytes > (size_t)0x7fffffff)
+        return ctx->fail(HP3D_ERR_UNSUPPORTED, "image %dx%d: plane above 2 GiB", W, H);
+    int rc = ctx->dev_alloc(&img.plain, img.plain_bytes, "image plane");
     if (rc) return rc;
     rc = ctx->dev_alloc(&img.coef, img.coef_bytes, "image coefficient plane");
     if (rc) { ctx->dev_free(img.plain, img.plain_bytes); return rc; }
+    auto release = [&] {
+        ctx->dev_free(img.plain, img.plain_bytes);
+        ctx->dev_free(img.coef, img.coef_bytes);
+    };
 
     // stage the host rows on the device, then convert in one kernel
-    void* stage = nullptr;
     const size_t stage_bytes = row * H;
     rc = ctx->grow(ctx->staging, stage_bytes, "upload staging");
-    if (rc) { ctx->dev_free(img.plain, img.plain_bytes); ctx->dev_free(img.coef, img.coef_bytes); return rc; }
-    stage = ctx->staging.ptr;
+    if (rc) { release(); return rc; }
+    void* stage = ctx->staging.ptr;
     hipError_t e = hipMemcpy2DAsync(stage, row, data, pitch, row, H, hipMemcpyHostToDevice,
                                     ctx->stream);
     if (e == hipSuccess)
         e = launch_img_prep(fmt == HP3D_FMT_U8 ? (const uint8_t*)stage : nullptr,
                             fmt == HP3D_FMT_F32 ? (const float*)stage : nullptr, row, W, H, store,
-                            img.plain, img.pitch_plain, img.coef, img.pitch_coef, ctx->stream);
+                            img.plain, img.coef, img.pitch_plain, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
-        ctx->dev_free(img.plain, img.plain_bytes);
-        ctx->dev_free(img.coef, img.coef_bytes);
+        release();
         return ctx->hip_fail(e, "image upload");
     }
     const hp3d_img h = ++ctx->next_handle;

@@ -2,16 +2,16 @@
 // the PatchMatch-MVS path.
 //
 // Data layout in HBM (one reference view being solved):
-//   * images: every uploaded image keeps two planes, both shifted by -128 DN
-//     so that window sums stay small (the NCC is invariant to the shift):
-//       plain[y][x]        = DN - 128                 (reference windows)
-//       coef [y][x] (x<W-1, y<H-1) = (a, b-a, c-a, a-b-c+d), a = plain[y][x],
-//         b = plain[y][x+1], c = plain[y+1][x], d = plain[y+1][x+1]
-//     so that a bilinear sample at (x+fx, y+fy) is
-//       a + fx*(b-a) + fy*((c-a) + fx*(a-b-c+d))   -- one load, three FMAs.
-//     Element type is float (F32 storage) or _Float16 (F16 storage).  For
-//     8-bit input every coefficient is an integer in [-510, 510], so F16
-//     storage is lossless.
+//   * images: plain[y][x] = DN - 128, row pitch a multiple of 64 elements.
+//     The shift keeps window sums small (the NCC is invariant to it).
+//     F32 storage: the float plane only (4 B/px); a bilinear tap reads the
+//     texel pairs (x, x+1) of rows y and y+1 with one 8-byte load each.
+//     F16 storage: a _Float16 plane (reference windows) plus a coefficient
+//     plane coef[y][x] = (a, b-a, c-a, (a-b)-(c-d)) of the quad
+//     a = plain[y][x], b = plain[y][x+1], c = plain[y+1][x],
+//     d = plain[y+1][x+1] as four halves (10 B/px in all); a tap is one
+//     8-byte load.  For 8-bit input every value is an integer in
+//     [-510, 510], so F16 storage is lossless.
 //   * hypotheses: two compacted colour planes (red: (x+y) even, black: odd),
 //     each H x Wc with Wc = (W+1)/2.  Pixel (x, y) of colour c sits at
 //     column i = x >> 1 of plane c.  hyp = (depth, nx, ny, nz) as float4,
@@ -36,11 +36,12 @@ struct DevSrc {
 };
 
 struct DevView {
-    const void* ref_plain;              // reference plain plane
-    const void* src_coef[kMaxSrc];      // source bilinear-coefficient planes
-    int pitch_plain;                    // elements per row of a plain plane
-    int pitch_coef;                     // elements (texels) per row of a coef plane
-    int coef_bytes;                     // bytes of one coef plane (buffer range check)
+    const void* ref_plain;              // reference plane
+    const void* src_plain[kMaxSrc];     // source planes (F32 storage)
+    const void* src_coef[kMaxSrc];      // source coefficient planes (F16 storage)
+    int pitch_plain;                    // elements per row of a plane (both kinds)
+    int plain_bytes;                    // bytes of one plane (buffer range check)
+    int coef_bytes;                     // bytes of one coefficient plane
     float kinv[9];                      // K_ref^-1 (row-major), hypothesis arithmetic
     double kinv_d[9];                   // K_ref^-1 for the cost
     DevSrc src[kMaxSrc];

@@ -13,8 +13,8 @@
 //     radius-wide halo, clamped at the image edge) in LDS once;
 //   * per candidate and source the plane homography H = A + b (K^-T n)^T / (n.P)
 //     is formed once (window centre in FP64), so a tap costs 3 FMAs for the
-//     warp, one v_rcp_f32, one buffer load of the 4 bilinear coefficients and
-//     3 FMAs for the sample;
+//     warp, one v_rcp_f32, two buffer loads (the texel pairs of two rows) and
+//     6 FMAs for the bilinear sample;
 //   * the five NCC moments are accumulated in FP32 on values shifted by
 //     -128 DN; the variance cancellation is done in FP64.
 #include <hip/hip_runtime.h>
@@ -82,12 +82,13 @@ __global__ __launch_bounds__(256) void pm_import_kernel(DevView v, const float* 
         make_float4(depth[q], normal[3 * q], normal[3 * q + 1], normal[3 * q + 2]);
 }
 
-// 8-bit or [0,1] float gray -> plain and bilinear-coefficient planes (see
-// pm_device.h).  in_u8 / in_f32: exactly one is non-null; in_pitch in bytes.
+// 8-bit or [0,1] float gray -> the image planes (DN - 128, see pm_device.h).
+// in_u8 / in_f32: exactly one is non-null; in_pitch in bytes.  coef is used
+// by F16 storage only.
 template <int STORE>
 __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, const float* in_f32,
                                                        size_t in_pitch, int W, int H, void* plain,
-                                                       int pitch_plain, void* coef, int pitch_coef)
+                                                       void* coef, int pitch)
 {
     const int x = blockIdx.x * 64 + (threadIdx.x & 63);
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -98,17 +99,18 @@ __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, con
         return row[xx] * 255.0f - kShiftDN;
     };
     const float a = val(x, y);
-    if (STORE == kStoreF32) static_cast<float*>(plain)[(size_t)y * pitch_plain + x] = a;
-    else static_cast<_Float16*>(plain)[(size_t)y * pitch_plain + x] = (_Float16)a;
-    if (x < W - 1 && y < H - 1) {
-        const float b = val(x + 1, y), c = val(x, y + 1), d = val(x + 1, y + 1);
-        const float cb = b - a, cc = c - a, cd = (a - b) - (c - d);
-        const size_t q = (size_t)y * pitch_coef + x;
-        if (STORE == kStoreF32) {
-            static_cast<float4*>(coef)[q] = make_float4(a, cb, cc, cd);
-        } else {
+    const size_t q = (size_t)y * pitch + x;
+    if (STORE == kStoreF32) {
+        static_cast<float*>(plain)[q] = a;
+    } else {
+        static_cast<_Float16*>(plain)[q] = (_Float16)a;
+        if (x < W - 1 && y < H - 1) {
+            const float b = val(x + 1, y), c = val(x, y + 1), d = val(x + 1, y + 1);
             half4_t h;
-            h.x = (_Float16)a; h.y = (_Float16)cb; h.z = (_Float16)cc; h.w = (_Float16)cd;
+            h.x = (_Float16)a;
+            h.y = (_Float16)(b - a);
+            h.z = (_Float16)(c - a);
+            h.w = (_Float16)((a - b) - (c - d));
             static_cast<half4_t*>(coef)[q] = h;
         }
     }
@@ -141,15 +143,14 @@ hipError_t launch_import(const DevView& v, const float* depth, const float* norm
 }
 
 hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_pitch, int W,
-                           int H, int store, void* plain, int pitch_plain, void* coef,
-                           int pitch_coef, hipStream_t st)
+                           int H, int store, void* plain, void* coef, int pitch, hipStream_t st)
 {
     if (store == kStoreF32)
         hipLaunchKernelGGL(img_prep_kernel<kStoreF32>, grid_2d(W, H), dim3(256), 0, st, in_u8,
-                           in_f32, in_pitch, W, H, plain, pitch_plain, coef, pitch_coef);
+                           in_f32, in_pitch, W, H, plain, coef, pitch);
     else
         hipLaunchKernelGGL(img_prep_kernel<kStoreF16>, grid_2d(W, H), dim3(256), 0, st, in_u8,
-                           in_f32, in_pitch, W, H, plain, pitch_plain, coef, pitch_coef);
+                           in_f32, in_pitch, W, H, plain, coef, pitch);
     return hipGetLastError();
 }
 

@@ -15,9 +15,9 @@ hipError_t launch_finalize(const DevView& v, float* depth, float* normal, float*
                            hipStream_t st);
 hipError_t launch_import(const DevView& v, const float* depth, const float* normal,
                          hipStream_t st);
+// coef: the F16 coefficient plane (nullptr for F32 storage); both planes use `pitch`.
 hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_pitch, int W,
-                           int H, int store, void* plain, int pitch_plain, void* coef,
-                           int pitch_coef, hipStream_t st);
+                           int H, int store, void* plain, void* coef, int pitch, hipStream_t st);
 // mode kPassCost: cost of the current hypotheses of colour `parity`;
 // mode kPassPropagate: one half-pass of colour `parity` at `iteration`.
 hipError_t launch_pass(const DevView& v, int store, int radius, int stride, int mode,

@@ -13,15 +13,25 @@
 
 namespace hp3d {
 
-// One uploaded image: plain + bilinear-coefficient planes (pm_device.h).
+// One uploaded image: its plane of DN - 128 values and, for F16 storage, the
+// bilinear-coefficient plane (pm_device.h).
 struct Image {
     int W = 0, H = 0, store = 0;
-    int pitch_plain = 0, pitch_coef = 0;   // elements
+    int pitch_plain = 0;                   // elements, both planes
     void* plain = nullptr;
     void* coef = nullptr;
     size_t plain_bytes = 0, coef_bytes = 0;
     float den_scale = 1.0f;                // stored-unit^2 per gray^2
 };
+
+// Plane sizes of a W x H image with the given storage.
+inline void image_plane_sizes(int W, int H, int store, int* pitch, size_t* plain_bytes,
+                              size_t* coef_bytes)
+{
+    *pitch = (W + 63) & ~63;
+    *plain_bytes = (size_t)*pitch * H * (store == HP3D_STORE_F32 ? 4 : 2);
+    *coef_bytes = store == HP3D_STORE_F32 ? 0 : (size_t)*pitch * H * 8;
+}
 
 struct DevBuf {
     void* ptr = nullptr;

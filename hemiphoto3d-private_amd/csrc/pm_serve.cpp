@@ -128,12 +128,12 @@ struct hp3d_pm_server {
 
 size_t hp3d_pm_server::footprint(const hp3d_job_desc& d) const
 {
-    const size_t esz = o.store == HP3D_STORE_F32 ? 4 : 2;
-    const size_t pitch = (size_t)((d.width + 63) & ~63);
+    int pitch;
+    size_t plain_bytes, coef_bytes;
+    image_plane_sizes(d.width, d.height, o.store, &pitch, &plain_bytes, &coef_bytes);
     const size_t n = 1 + d.num_src;
     const size_t mask = 1 | (d.out_normal ? 2 : 0) | (d.out_cost ? 4 : 0);
-    return a256(n * (size_t)d.width * d.height) + n * (a256(pitch * d.height * esz) +
-                                                        a256(pitch * d.height * esz * 4)) +
+    return a256(n * (size_t)d.width * d.height) + n * (a256(plain_bytes) + a256(coef_bytes)) +
            a256(job_state_bytes(d.width, d.height, (int)mask));
 }
 
@@ -141,8 +141,6 @@ size_t hp3d_pm_server::footprint(const hp3d_job_desc& d) const
 void hp3d_pm_server::layout(SJob& j)
 {
     const hp3d_job_desc& d = j.d;
-    const size_t esz = o.store == HP3D_STORE_F32 ? 4 : 2;
-    const int pitch = (d.width + 63) & ~63;
     char* p = arena_base + j.off;
     p += a256((size_t)(1 + d.num_src) * d.width * d.height);          // staging first
     for (int k = 0; k <= d.num_src; ++k) {
@@ -150,13 +148,12 @@ void hp3d_pm_server::layout(SJob& j)
         im.W = d.width;
         im.H = d.height;
         im.store = o.store;
-        im.pitch_plain = im.pitch_coef = pitch;
-        im.plain_bytes = (size_t)pitch * d.height * esz;
-        im.coef_bytes = im.plain_bytes * 4;
+        image_plane_sizes(d.width, d.height, o.store, &im.pitch_plain, &im.plain_bytes,
+                          &im.coef_bytes);
         im.den_scale = 255.0f * 255.0f;
         im.plain = p;
         p += a256(im.plain_bytes);
-        im.coef = p;
+        im.coef = im.coef_bytes ? p : nullptr;
         p += a256(im.coef_bytes);
     }
     hp3d_view_desc vd{};
@@ -301,8 +298,7 @@ void hp3d_pm_server::loop()
                 for (int k = 0; k <= j->d.num_src && ue == hipSuccess; ++k)
                     ue = launch_img_prep((const uint8_t*)staging + (size_t)k * W * H, nullptr,
                                          (size_t)W, W, H, o.store, j->imgs[k].plain,
-                                         j->imgs[k].pitch_plain, j->imgs[k].coef,
-                                         j->imgs[k].pitch_coef, upload);
+                                         j->imgs[k].coef, j->imgs[k].pitch_plain, upload);
                 if (ue == hipSuccess) ue = hipEventCreateWithFlags(&j->ready_ev, hipEventDisableTiming);
                 if (ue == hipSuccess) ue = hipEventRecord(j->ready_ev, upload);
                 j->st.state = HP3D_JOB_RUNNING;
@@ -401,10 +397,12 @@ int hp3d_server_submit(hp3d_pm_server* s, const hp3d_job_desc* d, uint64_t* id)
     auto j = std::make_shared<SJob>();
     j->d = *d;
     j->footprint = s->footprint(*d);
-    // taps address a coefficient plane with 32-bit buffer offsets
-    const size_t coef_bytes = (size_t)((d->width + 63) & ~63) * d->height *
-                              (s->o.store == HP3D_STORE_F32 ? 16 : 8);
-    if (j->footprint > s->arena.size() || coef_bytes > 0x7fffffffull) {
+    // taps address an image plane with 32-bit buffer offsets
+    int pitch;
+    size_t plain_bytes, coef_bytes;
+    image_plane_sizes(d->width, d->height, s->o.store, &pitch, &plain_bytes, &coef_bytes);
+    if (j->footprint > s->arena.size() || plain_bytes > 0x7fffffffull ||
+        coef_bytes > 0x7fffffffull) {
         s->last_error = "submit: job footprint exceeds the server's HBM budget";
         s->stats.rejected++;
         return HP3D_ERR_OOM_BUDGET;

@@ -25,10 +25,11 @@ def serve_jobs(scenes):
 
 @pytest.fixture(scope="module")
 def expected(pkg, ctx, serve_jobs):
+    """Per-view solves with the server's storage (FP16)."""
     out = []
     for job in serve_jobs:
-        ref = ctx.upload(job.ref)
-        srcs = [ctx.upload(s) for s in job.srcs]
+        ref = ctx.upload(job.ref, pkg.STORE_F16)
+        srcs = [ctx.upload(s, pkg.STORE_F16) for s in job.srcs]
         d = pkg.make_desc(ref, srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax, radius=5,
                           stride=2, iterations=job.iterations, seed=job.seed)
         out.append(ctx.run_view(d, job.shape))
