@@ -45,6 +45,14 @@ def parse():
     ap.add_argument("--scale", type=float, default=1.0, help="shrink W, H (not for reporting)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
+    ap.add_argument("--form", choices=["plain", "precision", "batched", "serving"],
+                    default="plain",
+                    help="plain: the headline line; the others print their own JSON line")
+    ap.add_argument("--views", type=int, default=48, help="batched: views per batch")
+    ap.add_argument("--rate", type=float, default=20.0, help="serving: arrivals per second")
+    ap.add_argument("--duration", type=float, default=120.0, help="serving: arrival window (s)")
+    ap.add_argument("--stride", type=int, default=None,
+                    help="batched/serving window stride (default: 1 batched, 2 serving)")
     return ap.parse_args()
 
 
@@ -77,6 +85,10 @@ def main():
     from importlib import import_module
     pkg = entry.load_package()
     scenes = import_module(entry.PKG_NAME + ".scenes")
+    if a.form != "plain":
+        {"precision": form_precision, "batched": form_batched,
+         "serving": form_serving}[a.form](a, entry, pkg, scenes)
+        return
 
     job = scenes.c2(threads=a.cpu_threads, scale=a.scale)
     H, W = job.shape
@@ -172,6 +184,267 @@ def cpu_baseline(entry, job, threads, k):
                       f"{dt:.1f} s; scaled by the tap rate to the {job.iterations}-iteration workload",
             "taps_per_s": round(taps_per_s)}
 
+
+
+# ---------------------------------------------------------------------------
+# The other forms (each prints one JSON line of its own)
+# ---------------------------------------------------------------------------
+def _taps_per_launch(job):
+    n = (2 * job.radius) // job.stride + 1
+    H, W = job.shape
+    return (W * H / 2.0) * 6 * job.S * n * n
+
+
+def _cpu_tap_rate(entry, threads):
+    """CPU baseline tap rate: the oracle's FP32 restatement on a 640x480
+    4-source view for 1 iteration (a few seconds), in taps per second."""
+    from importlib import import_module
+    scenes = import_module(entry.PKG_NAME + ".scenes")
+    orc = entry.load_oracle()
+    job = scenes.sphere_job(640, 480, 4, radius=5, stride=1, iterations=1, threads=threads)
+    v = orc.View(job.ref, job.srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax, radius=5,
+                 stride=1, top_k=3)
+    t0 = time.perf_counter()
+    v.patchmatch_fp32(1, job.seed, threads=threads, cost_mode=orc.COST_FP32)
+    return job.taps() / (time.perf_counter() - t0)
+
+
+def _device_outputs(ctx, shape):
+    H, W = shape
+    return tuple(ctx.device_alloc(W * H * 4 * m) for m in (1, 3, 1))
+
+
+def form_precision(a, entry, pkg, scenes):
+    """3200x2400 view, 6 sources at +-5/10/15 deg, stride 2, 8 iterations:
+    FP16 image storage (FP32 accumulation, FP64 host homographies) against
+    the FP32 plain storage on the same view."""
+    job = scenes.c3(threads=a.cpu_threads)
+    H, W = job.shape
+    ctx = pkg.Context(timing=True)
+    res = {}
+    for name, store in (("fp32", pkg.STORE_F32), ("fp16", pkg.STORE_F16)):
+        ref = ctx.upload(job.ref, store)
+        srcs = [ctx.upload(s, store) for s in job.srcs]
+        desc = pkg.make_desc(ref, srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                             radius=job.radius, stride=job.stride, iterations=job.iterations,
+                             seed=job.seed)
+        outs = _device_outputs(ctx, job.shape)
+        ctx.run_view_device(desc, *outs)
+        steps, prop_ms, prop_n = max(2, a.steps // 4), 0.0, 0
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            ctx.run_view_device(desc, *outs)
+            st = ctx.stats()
+            prop_ms += st["ms_propagate"]
+            prop_n += st["n_propagate"]
+        dt = (time.perf_counter() - t0) / steps
+        depth = ctx.download(outs[0], (H, W))
+        achieved = _taps_per_launch(job) * FLOP_PER_TAP / (prop_ms / prop_n / 1e3) / 1e12
+        res[name] = {"mpix_iters_per_s": round(job.mpix_iters() / dt, 2),
+                     "ms_per_view": round(dt * 1e3, 3),
+                     "halfpass_tflops": round(achieved, 3),
+                     "gt_within_1pct": round(float((np.abs(depth - job.gt_depth)
+                                                    <= 0.01 * job.gt_depth).mean()), 4),
+                     "depth": depth}
+        for p in outs:
+            ctx.device_free(p)
+        ctx.release(ref)
+        for s in srcs:
+            ctx.release(s)
+    d32, d16 = res["fp32"].pop("depth"), res["fp16"].pop("depth")
+    agree = float((np.abs(d16 - d32) <= 0.01 * np.abs(d32)).mean())
+    out = {"form": "precision", "metric": "PatchMatch Mpixel-iters/s",
+           "value": res["fp16"]["mpix_iters_per_s"], "unit": "Mpixel-iters/s",
+           "dtype": "fp16 storage / fp32 accumulation", "data": "synthetic",
+           "config": {"workload": "precision form: 1 ref + 6 src, 3200x2400 uint8, 11x11 stride 2, "
+                                  "depth [2,12], 8 iterations", "width": W, "height": H,
+                      "sources": job.S},
+           "fp16": res["fp16"], "fp32": res["fp32"],
+           "speedup_fp16_vs_fp32": round(res["fp16"]["mpix_iters_per_s"]
+                                         / res["fp32"]["mpix_iters_per_s"], 3),
+           "depth_within_1pct_fp16_vs_fp32": round(agree, 5),
+           "roofline": {"bound": "valu", "achieved": res["fp16"]["halfpass_tflops"],
+                        "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(res["fp16"]["halfpass_tflops"] / PEAK_FP32_TFLOPS, 4)}}
+    print(json.dumps(out))
+    ctx.close()
+
+
+def form_batched(a, entry, pkg, scenes):
+    """48 views, resolutions uniform over {640x480 ... 4000x3000}, 3-8 sources,
+    11x11 window, 8 iterations, FP16 storage, one hipGraph per batch, tile
+    shape chosen by the autotuner; against the plain FP32 kernel run view by
+    view over the same views."""
+    stride = a.stride or 1
+    specs = scenes.batch_specs(a.views, iterations=8, stride=stride)
+    t0 = time.perf_counter()
+    jobs = [scenes.job_from_spec(s, threads=a.cpu_threads) for s in specs]
+    render_s = time.perf_counter() - t0
+    ctx = pkg.Context(timing=True)
+    mpix = sum(j.mpix_iters() for j in jobs)
+    taps = sum(j.taps() for j in jobs)
+
+    def descs_for(store):
+        out = []
+        for j in jobs:
+            ref = ctx.upload(j.ref, store)
+            srcs = [ctx.upload(s, store) for s in j.srcs]
+            out.append(pkg.make_desc(ref, srcs, j.Kinv, j.A, j.b, j.dmin, j.dmax, radius=j.radius,
+                                     stride=j.stride, iterations=j.iterations, seed=j.seed))
+        return out
+
+    d16 = descs_for(pkg.STORE_F16)
+    outs = [_device_outputs(ctx, j.shape) for j in jobs]
+    best, times = ctx.autotune(d16)
+    ctx.batch_run_device(d16, outs)                       # warm-up (captures the graph)
+    reps = 2
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        ctx.batch_run_device(d16, outs)
+    batched_s = (time.perf_counter() - t0) / reps
+    # plain kernel, FP32 storage, one view after the other
+    d32 = descs_for(pkg.STORE_F32)
+    ctx.run_view_device(d32[0], *outs[0])
+    t0 = time.perf_counter()
+    for d, o in zip(d32, outs):
+        ctx.run_view_device(d, *o)
+    plain_s = time.perf_counter() - t0
+    cpu = None
+    if not a.no_cpu:
+        rate = _cpu_tap_rate(entry, a.cpu_threads)
+        cpu = {"value": round(mpix / (taps / rate), 3), "unit": "Mpixel-iters/s",
+               "cores": a.cpu_threads, "kind": "port",
+               "sample": "640x480 4-source view, 1 iteration, extrapolated by taps to the batch"}
+    import ctypes
+    ci = [ctypes.c_int() for _ in range(3)]
+    pkg.lib().hp3d_pm_variant_info(best, *[ctypes.byref(c) for c in ci])
+    value = mpix / batched_s
+    out = {"form": "batched", "metric": "PatchMatch Mpixel-iters/s", "value": round(value, 2),
+           "unit": "Mpixel-iters/s", "dtype": "fp16 storage / fp32 accumulation",
+           "data": "synthetic", "config": {
+               "workload": f"batched form: {a.views} views, resolutions over "
+                           "{640x480,1280x960,1600x1200,2400x1800,4000x3000}, 3-8 sources, "
+                           f"11x11 stride {stride}, 8 iterations, one graph launch",
+               "views": a.views, "mpixel_iters": round(mpix, 2), "taps": taps,
+               "render_s": round(render_s, 1)},
+           "ms_per_batch": round(batched_s * 1e3, 2),
+           "plain_serial_value": round(mpix / plain_s, 2),
+           "speedup_vs_plain_serial": round(plain_s / batched_s, 3),
+           "autotune": {"variant": best, "tile": [ci[0].value, ci[1].value],
+                        "waves_per_simd_hint": ci[2].value,
+                        "times_ms": [round(float(t), 3) for t in times]},
+           "tflops_algorithmic": round(taps * FLOP_PER_TAP / batched_s / 1e12, 3),
+           "cpu_baseline": cpu}
+    if cpu:
+        out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
+    print(json.dumps(out))
+    ctx.close()
+
+
+def form_serving(a, entry, pkg, scenes):
+    """Poisson arrivals (rate, duration); each job 1 ref + 3-8 sources at a
+    resolution of the batched mix, 4-12 iterations, FP16 storage, continuous
+    batching with HBM-budgeted admission.  Latency = completion - submission."""
+    stride = a.stride or 2
+    rng = np.random.default_rng(2024)
+    pool_specs = scenes.batch_specs(16, batch_seed=77, iterations=8, stride=stride)
+    pool = [scenes.job_from_spec(s, threads=a.cpu_threads) for s in pool_specs]
+    n_arr = rng.poisson(a.rate * a.duration)
+    arrivals = np.sort(rng.uniform(0, a.duration, n_arr))
+    picks = rng.integers(0, len(pool), n_arr)
+    iters = rng.integers(4, 13, n_arr)
+    offered = sum(pool[p].shape[0] * pool[p].shape[1] * it for p, it in zip(picks, iters)) / 1e6 / a.duration
+    srv = pkg.Server(hbm_budget=96 << 30, max_jobs=64, radius=5, stride=stride,
+                     store=pkg.STORE_F16)
+    for w in range(2):                                   # warm-up
+        srv.wait(srv.submit_job(pool[w], iterations=4), timeout=120)
+    lat, done_mpix, t_first, t_last = [], 0.0, None, None
+    pending = []
+
+    def collect(st, p, it):
+        nonlocal done_mpix, t_first, t_last
+        if st is None or st["state"] != pkg.JOB_DONE:
+            return
+        lat.append(st["t_done"] - st["t_submit"])
+        H, W = pool[p].shape
+        done_mpix += W * H * it / 1e6
+        t_first = st["t_submit"] if t_first is None else min(t_first, st["t_submit"])
+        t_last = st["t_done"] if t_last is None else max(t_last, st["t_done"])
+
+    t0, dropped, max_queue, failed = time.perf_counter(), 0, 0, 0
+    for k in range(n_arr):
+        now = time.perf_counter() - t0
+        if arrivals[k] > now:
+            time.sleep(arrivals[k] - now)
+        # collect finished jobs so host output buffers do not pile up
+        keep = []
+        for jid, p, it in pending:
+            st = srv.poll(jid)
+            if st["state"] in (pkg.JOB_QUEUED, pkg.JOB_RUNNING):
+                keep.append((jid, p, it))
+            else:
+                failed += st["state"] != pkg.JOB_DONE
+                collect(st, p, it)
+                srv.forget(jid)
+        pending = keep
+        max_queue = max(max_queue, len(pending))
+        if len(pending) > 400:                           # host memory guard when overloaded
+            dropped += 1
+            continue
+        pending.append((srv.submit_job(pool[picks[k]], iterations=int(iters[k]),
+                                       seed=1000 + k, with_cost=False),
+                        int(picks[k]), int(iters[k])))
+        if k % 200 == 0:
+            print(f"serving: {k}/{n_arr} submitted at t={now:.1f}s, in flight {len(pending)}",
+                  file=sys.stderr, flush=True)
+    for jid, p, it in pending:
+        st = srv.wait(jid, timeout=600)
+        failed += st is None or st["state"] != pkg.JOB_DONE
+        collect(st, p, it)
+        srv.forget(jid)
+    s = srv.stats()
+    srv.close()
+    wall = t_last - t_first
+    # the plain FP32 kernel, one job after the other, on the pool at 8 iterations
+    ctx = pkg.Context()
+    plain_mpix, plain_s = 0.0, 0.0
+    for j in pool:
+        ref = ctx.upload(j.ref)
+        srcs = [ctx.upload(x) for x in j.srcs]
+        d = pkg.make_desc(ref, srcs, j.Kinv, j.A, j.b, j.dmin, j.dmax, radius=5, stride=stride,
+                          iterations=8, seed=j.seed)
+        outs = _device_outputs(ctx, j.shape)
+        ctx.run_view_device(d, *outs)
+        t1 = time.perf_counter()
+        ctx.run_view_device(d, *outs)
+        plain_s += time.perf_counter() - t1
+        plain_mpix += j.mpix_iters()
+        for q in outs:
+            ctx.device_free(q)
+        ctx.release(ref)
+        for x in srcs:
+            ctx.release(x)
+    ctx.close()
+    lat = np.array(lat) * 1e3
+    value = done_mpix / wall
+    out = {"form": "serving", "metric": "PatchMatch Mpixel-iters/s; p50/p99 job latency",
+           "value": round(value, 2), "unit": "Mpixel-iters/s",
+           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                          "p99": round(float(np.percentile(lat, 99)), 2),
+                          "mean": round(float(lat.mean()), 2)},
+           "dtype": "fp16 storage / fp32 accumulation", "data": "synthetic",
+           "config": {"workload": f"serving form: Poisson {a.rate}/s for {a.duration}s, 1 ref + "
+                                  "3-8 src from the batched resolution mix, 4-12 iterations, "
+                                  f"11x11 stride {stride}", "pool_jobs": len(pool)},
+           "arrivals": int(n_arr), "completed": len(lat), "failed": int(failed),
+           "dropped_overload": dropped,
+           "offered_mpix_iters_per_s": round(offered, 2),
+           "utilisation": round(s["busy_ms"] / 1e3 / s["wall_s"], 3),
+           "max_in_flight": int(max_queue), "max_active_per_step": s["max_active"],
+           "plain_serial_value": round(plain_mpix / plain_s, 2),
+           "speedup_vs_plain_serial": round(value / (plain_mpix / plain_s), 3),
+           "steps": s["steps"]}
+    print(json.dumps(out))
 
 if __name__ == "__main__":
     main()

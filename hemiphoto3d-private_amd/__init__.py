@@ -25,7 +25,7 @@ ERR_NOT_FOUND, ERR_UNSUPPORTED, ERR_IO, ERR_AGAIN = -5, -6, -7, -8
 FMT_U8, FMT_F32 = 0, 1
 STORE_F32, STORE_F16 = 0, 1
 CTX_TIMING = 1
-SCENE_PLANE, SCENE_SPHERE_GROUND = 0, 1
+SCENE_PLANE, SCENE_SPHERE_GROUND, SCENE_SPHERE = 0, 1, 2
 
 _f32p = ctypes.POINTER(ctypes.c_float)
 _f64p = ctypes.POINTER(ctypes.c_double)
@@ -43,6 +43,11 @@ class Opts(ctypes.Structure):
                 ("stream", ctypes.c_void_p), ("flags", ctypes.c_int)]
 
 
+class SeedPoint(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("depth", ctypes.c_float),
+                ("normal", ctypes.c_float * 3)]
+
+
 class ViewDesc(ctypes.Structure):
     _fields_ = [
         ("ref", ctypes.c_uint64), ("src", ctypes.c_uint64 * MAX_SRC), ("num_src", ctypes.c_int),
@@ -52,7 +57,30 @@ class ViewDesc(ctypes.Structure):
         ("radius", ctypes.c_int), ("stride", ctypes.c_int), ("iterations", ctypes.c_int),
         ("top_k", ctypes.c_int), ("seed", ctypes.c_uint64),
         ("init_depth", _f32p), ("init_normal", _f32p),
+        ("seeds", ctypes.POINTER(SeedPoint)), ("num_seeds", ctypes.c_int),
+        ("seed_radius", ctypes.c_int),
     ]
+
+
+class SfmCamera(ctypes.Structure):
+    _fields_ = [("image_id", ctypes.c_int), ("is_registered", ctypes.c_int),
+                ("K", ctypes.c_double * 9), ("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3)]
+
+
+class SfmPoint(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_double * 3), ("image_ids", ctypes.POINTER(ctypes.c_int)),
+                ("num_image_ids", ctypes.c_int)]
+
+
+class SourceView(ctypes.Structure):
+    _fields_ = [("image_id", ctypes.c_int), ("score", ctypes.c_float),
+                ("baseline_deg", ctypes.c_float)]
+
+
+class ViewParams(ctypes.Structure):
+    _fields_ = [("ref_image_id", ctypes.c_int), ("num_src", ctypes.c_int),
+                ("src", SourceView * MAX_SRC), ("depth_min", ctypes.c_float),
+                ("depth_max", ctypes.c_float)]
 
 
 class ViewOut(ctypes.Structure):
@@ -180,6 +208,16 @@ SIGNATURES = {
     "hp3d_pm_device_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "hp3d_pm_device_copy": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p,
                                            ctypes.c_size_t, ctypes.c_int]),
+    "hp3d_select_source_views": (ctypes.c_int, [ctypes.POINTER(SfmCamera), ctypes.c_int,
+                                                ctypes.POINTER(SfmPoint), ctypes.c_int,
+                                                ctypes.c_int, ctypes.POINTER(ViewParams),
+                                                ctypes.POINTER(ctypes.c_int)]),
+    "hp3d_project_seed_points": (ctypes.c_int, [ctypes.POINTER(SfmCamera),
+                                                ctypes.POINTER(SfmPoint), ctypes.c_int,
+                                                ctypes.c_float, ctypes.c_float, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int,
+                                                ctypes.POINTER(SeedPoint),
+                                                ctypes.POINTER(ctypes.c_int)]),
     "hp3d_pfm_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p]),
     "hp3d_pfm_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), _f32p, ctypes.c_size_t]),
@@ -467,8 +505,65 @@ class Context:
         return cost
 
 
+# ---------------------------------------------------------------------------
+# Host view set-up (SfM cameras and points as plain arrays)
+# ---------------------------------------------------------------------------
+class SfM:
+    """Cameras (K, R, t world->camera, registered flag; image id = index) and
+    sparse points (position, observing image ids), packed for the C ABI."""
+
+    def __init__(self, cameras, points):
+        self.cams = (SfmCamera * max(1, len(cameras)))()
+        for i, c in enumerate(cameras):
+            cc = self.cams[i]
+            cc.image_id = i
+            cc.is_registered = int(c.get("registered", True))
+            for name in ("K", "R", "t"):
+                vals = np.asarray(c[name], np.float64).reshape(-1)
+                for j, v in enumerate(vals):
+                    getattr(cc, name)[j] = v
+        self.n_cams = len(cameras)
+        self.pts = (SfmPoint * max(1, len(points)))()
+        self._ids = []
+        for k, (pos, ids) in enumerate(points):
+            ids = np.ascontiguousarray(ids, dtype=np.int32)
+            self._ids.append(ids)
+            p = self.pts[k]
+            for j in range(3):
+                p.position[j] = float(pos[j])
+            p.image_ids = ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+            p.num_image_ids = len(ids)
+        self.n_pts = len(points)
+
+    def select_source_views(self, num_sources=5):
+        """selectSourceViews: list of dicts (ref_image_id, sources [(id, score, deg)],
+        depth_min, depth_max), one per registered camera."""
+        out = (ViewParams * max(1, self.n_cams))()
+        n = ctypes.c_int(0)
+        _check(lib().hp3d_select_source_views(self.cams, self.n_cams, self.pts, self.n_pts,
+                                              int(num_sources), out, ctypes.byref(n)))
+        res = []
+        for i in range(n.value):
+            v = out[i]
+            res.append({"ref_image_id": v.ref_image_id,
+                        "sources": [(v.src[s].image_id, v.src[s].score, v.src[s].baseline_deg)
+                                    for s in range(v.num_src)],
+                        "depth_min": v.depth_min, "depth_max": v.depth_max})
+        return res
+
+    def seed_points(self, ref_id, depth_min, depth_max, width, height, seed_radius=50):
+        """SfM seeds of camera ref_id: ctypes array of SeedPoint (and its length)."""
+        out = (SeedPoint * max(1, self.n_pts))()
+        n = ctypes.c_int(0)
+        _check(lib().hp3d_project_seed_points(ctypes.byref(self.cams[ref_id]), self.pts,
+                                              self.n_pts, float(depth_min), float(depth_max),
+                                              int(width), int(height), int(seed_radius), out,
+                                              ctypes.byref(n)))
+        return out, n.value
+
+
 def make_desc(ref, srcs, K_ref_inv, A, b, depth_min, depth_max, radius=5, stride=1,
-              iterations=0, top_k=3, seed=42, init=None):
+              iterations=0, top_k=3, seed=42, init=None, seeds=None, seed_radius=50):
     """Build a ViewDesc.  ref/srcs are image handles; A, b lists of FP64 terms.
 
     init: optional (depth (H,W), normal (H,W,3)) float32 host arrays replacing
@@ -498,6 +593,12 @@ def make_desc(ref, srcs, K_ref_inv, A, b, depth_min, depth_max, radius=5, stride
         d._keep = (idp, inn)
         d.init_depth = _p(idp, _f32p)
         d.init_normal = _p(inn, _f32p)
+    if seeds is not None:
+        arr, n = seeds
+        d._keep_seeds = arr
+        d.seeds = ctypes.cast(arr, ctypes.POINTER(SeedPoint))
+        d.num_seeds = int(n)
+        d.seed_radius = int(seed_radius)
     return d
 
 

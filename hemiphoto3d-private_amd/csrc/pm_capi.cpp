@@ -97,6 +97,19 @@ int hp3d::validate_desc(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image**
                          d->depth_max);
     if ((d->init_depth == nullptr) != (d->init_normal == nullptr))
         return ctx->fail(HP3D_ERR_INVALID_ARG, "init_depth and init_normal must be given together");
+    if (d->num_seeds < 0 || (d->num_seeds > 0 && !d->seeds))
+        return ctx->fail(HP3D_ERR_INVALID_ARG, "num_seeds %d without a seed array", d->num_seeds);
+    if (d->num_seeds > 0 && (d->seed_radius < 0 || d->seed_radius > kMaxSeedRadius))
+        return ctx->fail(HP3D_ERR_INVALID_ARG, "seed_radius %d outside 0..%d", d->seed_radius,
+                         kMaxSeedRadius);
+    for (int k = 0; k < d->num_seeds; ++k) {
+        const hp3d_seed_point& s = d->seeds[k];
+        if (s.x < -kMaxSeedCoord || s.x > kMaxSeedCoord || s.y < -kMaxSeedCoord ||
+            s.y > kMaxSeedCoord || !std::isfinite(s.depth) || !std::isfinite(s.normal[0]) ||
+            !std::isfinite(s.normal[1]) || !std::isfinite(s.normal[2]))
+            return ctx->fail(HP3D_ERR_INVALID_ARG, "seed %d: pixel (%d, %d) out of range or "
+                             "non-finite hypothesis", k, s.x, s.y);
+    }
     *ref = ctx->find(d->ref);
     if (!*ref) return ctx->fail(HP3D_ERR_NOT_FOUND, "unknown reference image handle %llu",
                                 (unsigned long long)d->ref);

@@ -23,6 +23,10 @@ namespace hp3d {
 
 constexpr int kMaxSrc = 8;
 constexpr int kMaxTopK = 8;
+// SfM seeds: half side of a seeded square, and |pixel coordinate| of a seed
+// (bounds that keep every square's corner arithmetic inside int).
+constexpr int kMaxSeedRadius = 4096;
+constexpr int kMaxSeedCoord = 1 << 24;
 constexpr float kShiftDN = 128.0f;
 
 enum Store : int { kStoreF32 = 0, kStoreF16 = 1 };

@@ -34,29 +34,50 @@ __global__ __launch_bounds__(256) void pm_init_kernel(DevView v, uint64_t seed)
     v.hyp[(x + y) & 1][y * v.Wc + (x >> 1)] = init_hyp(v, x, y, seed);
 }
 
+// Tile of this workgroup.  The grid is 1-D, a multiple of 8 blocks, and
+// workgroups are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8):
+// block b takes tile (b % 8) * (nblocks / 8) + b / 8 in row-major tile order,
+// so each XCD sweeps a contiguous band of the image and neighbouring tiles,
+// which gather from the same source rows, share that XCD's L2.  Returns
+// false for the padding blocks.
+__device__ __forceinline__ bool xcd_tile(int tiles_x, int tiles, int& bx, int& by)
+{
+    const int nb = gridDim.x, b = blockIdx.x;
+    const int t = (b & 7) * (nb >> 3) + (b >> 3);
+    if (t >= tiles) return false;
+    by = t / tiles_x;
+    bx = t - by * tiles_x;
+    return true;
+}
+
 // Cost of the current hypotheses of one colour, run once per colour before
 // the first iteration.
 template <int NT, int ST, int STORE, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void pm_cost_kernel(DevView v, int parity)
+__global__ __launch_bounds__(TX * TY) void pm_cost_kernel(DevView v, int parity, int tiles_x,
+                                                          int tiles)
 {
     constexpr int R = (NT - 1) * ST / 2;
     using G = TileGeom<R, STORE, TX, TY>;
     __shared__ float tile[G::LH * G::LP];
-    stage_ref_tile<R, STORE, TX, TY>(v, tile, blockIdx.x, blockIdx.y);
-    cost_pixel<NT, ST, STORE, TX, TY>(v, tile, blockIdx.x, blockIdx.y, threadIdx.x % TX,
-                                      threadIdx.x / TX, parity);
+    int bx, by;
+    if (!xcd_tile(tiles_x, tiles, bx, by)) return;
+    stage_ref_tile<R, STORE, TX, TY>(v, tile, bx, by);
+    cost_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, threadIdx.x % TX, threadIdx.x / TX, parity);
 }
 
 template <int NT, int ST, int STORE, int TX, int TY>
 __global__ __launch_bounds__(TX * TY) void pm_halfpass_kernel(DevView v, int iteration, int parity,
-                                                              uint64_t pass_seed)
+                                                              uint64_t pass_seed, int tiles_x,
+                                                              int tiles)
 {
     constexpr int R = (NT - 1) * ST / 2;
     using G = TileGeom<R, STORE, TX, TY>;
     __shared__ float tile[G::LH * G::LP];
-    stage_ref_tile<R, STORE, TX, TY>(v, tile, blockIdx.x, blockIdx.y);
-    halfpass_pixel<NT, ST, STORE, TX, TY>(v, tile, blockIdx.x, blockIdx.y, threadIdx.x % TX,
-                                          threadIdx.x / TX, iteration, parity, pass_seed);
+    int bx, by;
+    if (!xcd_tile(tiles_x, tiles, bx, by)) return;
+    stage_ref_tile<R, STORE, TX, TY>(v, tile, bx, by);
+    halfpass_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, threadIdx.x % TX, threadIdx.x / TX,
+                                          iteration, parity, pass_seed);
 }
 
 // Colour planes -> row-major depth (H*W), AoS normal (H*W*3), cost and
@@ -154,19 +175,23 @@ hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_
     return hipGetLastError();
 }
 
-constexpr int kTX = 16, kTY = 16;
+// 32 compacted columns (64 pixels) x 8 rows per workgroup: the fastest
+// shape of the step-kernel sweep on the 1600x1200 workload.
+constexpr int kTX = 32, kTY = 8;
 
 template <int NT, int ST, int STORE>
 static void launch_pass_t(const DevView& v, int mode, int iteration, int parity,
                           uint64_t pass_seed, hipStream_t st)
 {
-    const dim3 grid((v.Wc + kTX - 1) / kTX, (v.H + kTY - 1) / kTY);
+    const int tiles_x = (v.Wc + kTX - 1) / kTX;
+    const int tiles = tiles_x * ((v.H + kTY - 1) / kTY);
+    const dim3 grid(((tiles + 7) / 8) * 8);
     if (mode == kPassCost)
         hipLaunchKernelGGL((pm_cost_kernel<NT, ST, STORE, kTX, kTY>), grid, dim3(kTX * kTY), 0, st,
-                           v, parity);
+                           v, parity, tiles_x, tiles);
     else
         hipLaunchKernelGGL((pm_halfpass_kernel<NT, ST, STORE, kTX, kTY>), grid, dim3(kTX * kTY), 0,
-                           st, v, iteration, parity, pass_seed);
+                           st, v, iteration, parity, pass_seed, tiles_x, tiles);
 }
 
 template <int STORE>

@@ -3,6 +3,7 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "../../include/hp3d_pm.h"
 #include "pm_device.h"
 
 namespace hp3d {
@@ -18,6 +19,10 @@ hipError_t launch_import(const DevView& v, const float* depth, const float* norm
 // coef: the F16 coefficient plane (nullptr for F32 storage); both planes use `pitch`.
 hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_pitch, int W,
                            int H, int store, void* plain, void* coef, int pitch, hipStream_t st);
+// SfM seeds (device array of n) painted over the hypothesis planes; owner is
+// W*H ints of scratch.
+hipError_t launch_seed(const DevView& v, const hp3d_seed_point* seeds, int n, int radius,
+                       int* owner, hipStream_t st);
 // mode kPassCost: cost of the current hypotheses of colour `parity`;
 // mode kPassPropagate: one half-pass of colour `parity` at `iteration`.
 hipError_t launch_pass(const DevView& v, int store, int radius, int stride, int mode,

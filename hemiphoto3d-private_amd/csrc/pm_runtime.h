@@ -60,6 +60,7 @@ struct hp3d_pm_ctx {
 
     // per-view working set, grown on demand and reused across views
     hp3d::DevBuf hyp, cost, out_depth, out_normal, out_cost, in_depth, in_normal, staging;
+    hp3d::DevBuf seed_pts, seed_owner;
 
     // batched form: state arena, job table, step items, captured graph
     hp3d::DevBuf batch_state, batch_jobs, batch_items;
@@ -99,7 +100,8 @@ struct hp3d_pm_ctx {
             batch_graph = nullptr;
         }
         for (hp3d::DevBuf* b : {&hyp, &cost, &out_depth, &out_normal, &out_cost, &in_depth,
-                                &in_normal, &staging, &batch_state, &batch_jobs, &batch_items})
+                                &in_normal, &staging, &seed_pts, &seed_owner, &batch_state,
+                                &batch_jobs, &batch_items})
             release_buf(*b);
     }
 };

@@ -46,6 +46,23 @@ int import_hypotheses(hp3d_pm_ctx* ctx, const DevView& v, const float* depth, co
     return e == hipSuccess ? HP3D_OK : ctx->hip_fail(e, "hypothesis import");
 }
 
+// Paint the descriptor's SfM seeds over the current hypotheses.
+int apply_seeds(hp3d_pm_ctx* ctx, const DevView& v, const hp3d_view_desc* d)
+{
+    if (d->num_seeds <= 0) return HP3D_OK;
+    const size_t bytes = (size_t)d->num_seeds * sizeof(hp3d_seed_point);
+    int rc = ctx->grow(ctx->seed_pts, bytes, "seed points");
+    if (rc) return rc;
+    rc = ctx->grow(ctx->seed_owner, (size_t)v.W * v.H * sizeof(int), "seed owner map");
+    if (rc) return rc;
+    hipError_t e = hipMemcpyAsync(ctx->seed_pts.ptr, d->seeds, bytes, hipMemcpyHostToDevice,
+                                  ctx->stream);
+    if (e == hipSuccess)
+        e = launch_seed(v, static_cast<const hp3d_seed_point*>(ctx->seed_pts.ptr), d->num_seeds,
+                        d->seed_radius, static_cast<int*>(ctx->seed_owner.ptr), ctx->stream);
+    return e == hipSuccess ? HP3D_OK : ctx->hip_fail(e, "seeding");
+}
+
 // Finalize into device pointers, or into staging buffers followed by D2H copies.
 int emit_outputs(hp3d_pm_ctx* ctx, const DevView& v, float* depth, float* normal, float* cost,
                  float* conf, bool on_device)
@@ -112,6 +129,8 @@ int solve_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
         e = launch_init(v, d->seed, ctx->stream);
         if (e != hipSuccess) return ctx->hip_fail(e, "init launch");
     }
+    rc = apply_seeds(ctx, v, d);
+    if (rc) return rc;
     ctx->mark(kPhaseInit);
     for (int parity = 0; parity < 2 && e == hipSuccess; ++parity)
         e = launch_pass(v, ref->store, d->radius, d->stride, kPassCost, 0, parity, 0, ctx->stream);
@@ -142,6 +161,8 @@ int init_only(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
     if (rc) return rc;
     hipError_t e = launch_init(v, d->seed, ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "init launch");
+    rc = apply_seeds(ctx, v, d);
+    if (rc) return rc;
     rc = emit_outputs(ctx, v, depth, normal, nullptr, nullptr, false);
     if (rc) return rc;
     return finish(ctx, 0);

@@ -89,6 +89,7 @@ double intersect(const hp3d_scene& s, const double C[3], const double dir[3], do
                 for (int k = 0; k < 3; ++k) n[k] = (C[k] + t * dir[k] - c[k]) / s.sphere_radius;
             }
         }
+        if (s.kind == HP3D_SCENE_SPHERE) return best;
         const double gp[3] = {0.0, s.ground_y, 0.0}, gn[3] = {0.0, -1.0, 0.0};
         plane(gp, gn);
         const double bp[3] = {0.0, 0.0, s.backdrop_z}, bn[3] = {0.0, 0.0, -1.0};
@@ -143,7 +144,7 @@ extern "C" int hp3d_scene_render(const hp3d_scene* scene, const hp3d_camera* cam
             double nw[3] = {0, 0, 0};
             const double th = intersect(s, C, dir, nw);
             const size_t q = (size_t)y * W + x;
-            double v = 0.0;
+            double v = s.kind == HP3D_SCENE_SPHERE ? 30.0 : 0.0;
             double depth = 0.0, nc[3] = {0, 0, 0};
             if (th > 0.0) {
                 const double X[3] = {C[0] + th * dir[0], C[1] + th * dir[1], C[2] + th * dir[2]};

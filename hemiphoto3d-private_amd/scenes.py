@@ -230,3 +230,70 @@ def job_from_spec(spec, threads=0, name="batch"):
     return sphere_job(spec["W"], spec["H"], spec["S"], scene_seed=spec["scene_seed"],
                       stride=spec["stride"], iterations=spec["iterations"], seed=spec["seed"],
                       threads=threads, name=name)
+
+
+# ---------------------------------------------------------------------------
+# Turntable: the layout of the reference's own MVS test data
+# (src/utils/synthetic_data.cpp:17-28): 36 views 10 deg apart on a circle of
+# radius 0.25 around a textured sphere of radius 0.05, f = 2000 at
+# 1280x960, dark background.  The sparse "SfM" points are exact surface
+# points, each observed by the views that face it and see it in frame.
+# ---------------------------------------------------------------------------
+def turntable_scene(tex_seed=11):
+    pkg = _pkg()
+    s = pkg.Scene()
+    s.kind = pkg.SCENE_SPHERE
+    s.seed = tex_seed
+    s.octaves = 4
+    s.tex_freq = 400.0
+    for i, v in enumerate((0.0, 0.0, 0.25)):
+        s.sphere_center[i] = v
+    s.sphere_radius = 0.05
+    return s
+
+
+def turntable_cameras(n_views=36, step_deg=10.0, W=1280, H=960, f=2000.0):
+    K = pinhole(f, W, H)
+    return [dict(K=K, R=R, t=t) for R, t in
+            (arc_camera(step_deg * i, center_z=0.25, radius=0.25) for i in range(n_views))]
+
+
+def turntable_points(cams, n_points=3000, W=1280, H=960, min_cos=0.2):
+    """Fibonacci-lattice points on the sphere with their observing views."""
+    c = np.array([0.0, 0.0, 0.25])
+    r = 0.05
+    i = np.arange(n_points) + 0.5
+    phi = np.arccos(1 - 2 * i / n_points)
+    th = math.pi * (1 + 5 ** 0.5) * i
+    nrm = np.stack([np.cos(th) * np.sin(phi), np.cos(phi), np.sin(th) * np.sin(phi)], axis=1)
+    X = c + r * nrm
+    obs = [[] for _ in range(n_points)]
+    for v, cam in enumerate(cams):
+        R, t, K = cam["R"], cam["t"], cam["K"]
+        C = -R.T @ t
+        to_c = C - X
+        to_c /= np.linalg.norm(to_c, axis=1, keepdims=True)
+        facing = np.einsum("ij,ij->i", nrm, to_c) > min_cos
+        P = X @ R.T + t
+        uv = P @ K.T
+        u, w = uv[:, 0] / uv[:, 2], uv[:, 1] / uv[:, 2]
+        inside = (P[:, 2] > 0) & (u >= 0) & (u < W) & (w >= 0) & (w < H)
+        for k in np.nonzero(facing & inside)[0]:
+            obs[k].append(v)
+    return [(X[k], obs[k]) for k in range(n_points) if len(obs[k]) >= 2]
+
+
+def turntable(n_views=36, W=1280, H=960, f=2000.0, n_points=3000, noise_sigma=2.0,
+              threads=0):
+    """Images (uint8), GT depth maps, cameras and SfM points of the turntable."""
+    pkg = _pkg()
+    scene = turntable_scene()
+    cams = turntable_cameras(n_views, 360.0 / n_views, W, H, f)
+    imgs, gts = [], []
+    for v, cam in enumerate(cams):
+        img, gd, _ = pkg.render(scene, _camera(cam["K"], cam["R"], cam["t"], W, H),
+                                noise_sigma, 5000 + v, threads)
+        imgs.append(img)
+        gts.append(gd)
+    return dict(images=imgs, gt_depth=gts, cameras=cams,
+                points=turntable_points(cams, n_points, W, H))

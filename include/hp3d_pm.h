@@ -13,6 +13,8 @@
  *   init_depth_normal_kernel launch     .cu:890-902   -> hp3d_pm_init_hypotheses
  *   compute_ncc_cost_kernel launch      .cu:904-934   -> hp3d_pm_cost_eval
  *   selectSourceViews(sfm, n)           patchmatch_mvs.h:32-35 -> hp3d_select_source_views
+ *   SfM seeding in the driver           .cu:1111-1177  -> hp3d_project_seed_points
+ *                                                         + hp3d_view_desc.seeds
  *   saveDepthMapPFM / loadDepthMapPFM   patchmatch_mvs.h:59,62 -> hp3d_pfm_save / hp3d_pfm_load
  *
  * Conventions
@@ -83,6 +85,9 @@ typedef struct {
     uint64_t seed;                       /* view seed; reference: 42 + 1e6*ref_id (.cu:1106) */
     const float* init_depth;             /* optional host hypotheses replacing the random */
     const float* init_normal;            /*   init (H*W and H*W*3); both or neither */
+    const struct hp3d_seed_point* seeds; /* optional SfM seeds (hp3d_project_seed_points), */
+    int num_seeds;                       /*   applied after the init, before the first cost */
+    int seed_radius;                     /* half side of each seeded square; reference 50 */
 } hp3d_view_desc;
 
 typedef struct {
@@ -285,6 +290,66 @@ int hp3d_server_destroy(hp3d_pm_server* srv);
 int hp3d_pm_device_alloc(hp3d_pm_ctx* ctx, size_t bytes, void** dptr);
 int hp3d_pm_device_free(hp3d_pm_ctx* ctx, void* dptr);
 int hp3d_pm_device_copy(hp3d_pm_ctx* ctx, void* dst, const void* src, size_t bytes, int kind);
+
+/* ---------------------------------------------------------------------------
+ * Host view set-up: the CPU side of runPatchMatchMVS (patchmatch_mvs.cu:
+ * 1036-1039, 1111-1177) on plain arrays, for callers that hold an SfM result
+ * (SfMResult, src/sfm/camera_pose.h:8-14).
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int image_id;            /* must equal the camera's index in the array: the
+                                reference indexes sfm.cameras by image id (.cu:104) */
+    int is_registered;
+    double K[9], R[9], t[3]; /* world -> camera, row-major (include/types.h:55-74) */
+} hp3d_sfm_camera;
+
+typedef struct {
+    double position[3];      /* world point (SparsePoint, include/types.h:76-83) */
+    const int* image_ids;    /* views observing it */
+    int num_image_ids;
+} hp3d_sfm_point;
+
+typedef struct {             /* SourceViewInfo, patchmatch_mvs.h:16-20 */
+    int image_id;
+    float score;             /* shared points x sin(clamp(baseline, 5, 30 deg)) */
+    float baseline_deg;
+} hp3d_source_view;
+
+typedef struct {             /* MVSViewParams, patchmatch_mvs.h:23-28 */
+    int ref_image_id;
+    int num_src;
+    hp3d_source_view src[HP3D_PM_MAX_SRC];
+    float depth_min, depth_max;
+} hp3d_view_params;
+
+/* selectSourceViews (patchmatch_mvs.h:32-35, .cu:74-157): one entry per
+ * registered camera, in camera order, written to out (capacity num_cams);
+ * *num_out receives the count.  Sources: every other registered camera whose
+ * baseline angle at the SfM centroid is in [5, 90] deg and that shares at
+ * least one point, by descending score (ties: lower image id first; the
+ * reference's std::sort leaves tie order unspecified), at most num_sources
+ * (1..8).  Depth range: [0.5 P5, 1.5 P95] of the visible points' z, or
+ * [0.3 z_c, 3 z_c] at the centroid when fewer than 2 are visible. */
+int hp3d_select_source_views(const hp3d_sfm_camera* cams, int num_cams, const hp3d_sfm_point* pts,
+                             int num_pts, int num_sources, hp3d_view_params* out, int* num_out);
+
+/* One SfM seed: the pixel a sparse point projects to, its z-depth and the
+ * unit vector from the point to the camera centre in the camera frame. */
+typedef struct hp3d_seed_point {
+    int x, y;
+    float depth;
+    float normal[3];
+} hp3d_seed_point;
+
+/* Seeds of reference camera cam (.cu:1126-1155): every point observed by
+ * cam->image_id, in front of it, with depth in [depth_min, depth_max], in
+ * point order; points whose seeded square (half side seed_radius) misses the
+ * width x height image are left out, as they seed nothing.  out holds up to
+ * num_pts entries.  hp3d_view_desc.seeds applies them on the device with the
+ * reference's rule: a pixel takes the last seed whose square covers it. */
+int hp3d_project_seed_points(const hp3d_sfm_camera* cam, const hp3d_sfm_point* pts, int num_pts,
+                             float depth_min, float depth_max, int width, int height,
+                             int seed_radius, hp3d_seed_point* out, int* num_out);
 
 /* Portable FloatMap depth I/O, byte-compatible with saveDepthMapPFM /
  * loadDepthMapPFM (patchmatch_mvs.cu:990-1022): "Pf\n<W> <H>\n-1.0\n", rows

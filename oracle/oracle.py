@@ -42,6 +42,28 @@ class OrcGeomSrc(ctypes.Structure):
                 ("depth", _f32p), ("normal", _f32p)]
 
 
+class OrcCamera(ctypes.Structure):
+    _fields_ = [("image_id", ctypes.c_int), ("is_registered", ctypes.c_int),
+                ("K", ctypes.c_double * 9), ("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3)]
+
+
+class OrcPoint(ctypes.Structure):
+    _fields_ = [("position", ctypes.c_double * 3), ("image_ids", ctypes.POINTER(ctypes.c_int)),
+                ("num_image_ids", ctypes.c_int)]
+
+
+class OrcViewParams(ctypes.Structure):
+    _fields_ = [("ref_image_id", ctypes.c_int), ("num_src", ctypes.c_int),
+                ("src_id", ctypes.c_int * MAX_SRC), ("score", ctypes.c_float * MAX_SRC),
+                ("baseline_deg", ctypes.c_float * MAX_SRC), ("depth_min", ctypes.c_float),
+                ("depth_max", ctypes.c_float)]
+
+
+class OrcSeed(ctypes.Structure):
+    _fields_ = [("x", ctypes.c_int), ("y", ctypes.c_int), ("depth", ctypes.c_float),
+                ("normal", ctypes.c_float * 3)]
+
+
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
@@ -68,6 +90,9 @@ def lib():
         L.orc_cost_fp64.restype = ctypes.c_double
         L.orc_cost_map_fp64.argtypes = [ctypes.POINTER(OrcView), _f32p, _f32p, _f64p, _u32p,
                                         ctypes.c_int]
+        L.orc_cost_list_fp64.argtypes = [ctypes.POINTER(OrcView), ctypes.c_int,
+                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                         _f32p, _f32p, _f64p, _u32p, ctypes.c_int]
         L.orc_cost_fp32.argtypes = [ctypes.POINTER(OrcView), ctypes.c_int, ctypes.c_int,
                                     ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                     ctypes.c_float]
@@ -85,6 +110,16 @@ def lib():
                                            _f32p, _f32p, _f32p, ctypes.POINTER(OrcGeomSrc),
                                            ctypes.c_int, ctypes.c_float, ctypes.c_float,
                                            ctypes.c_int]
+        L.orc_select_source_views.argtypes = [ctypes.POINTER(OrcCamera), ctypes.c_int,
+                                              ctypes.POINTER(OrcPoint), ctypes.c_int,
+                                              ctypes.c_int, ctypes.POINTER(OrcViewParams)]
+        L.orc_select_source_views.restype = ctypes.c_int
+        L.orc_project_seeds.argtypes = [ctypes.POINTER(OrcCamera), ctypes.POINTER(OrcPoint),
+                                        ctypes.c_int, ctypes.c_float, ctypes.c_float,
+                                        ctypes.POINTER(OrcSeed)]
+        L.orc_project_seeds.restype = ctypes.c_int
+        L.orc_apply_seeds.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcSeed),
+                                      ctypes.c_int, ctypes.c_int, _f32p, _f32p]
         _lib = L
     return _lib
 
@@ -184,6 +219,21 @@ class View:
                                 _p(flags, _u32p), threads or os.cpu_count())
         return cost, flags
 
+    def cost_list_fp64(self, xs, ys, depth, normal, threads=None):
+        """FP64 cost at listed pixels; depth (n,), normal (n,3) per pixel."""
+        xs = np.ascontiguousarray(xs, dtype=np.int32)
+        ys = np.ascontiguousarray(ys, dtype=np.int32)
+        depth = np.ascontiguousarray(depth, dtype=np.float32)
+        normal = np.ascontiguousarray(normal, dtype=np.float32)
+        n = len(xs)
+        cost = np.zeros(n)
+        flags = np.zeros(n, dtype=np.uint32)
+        ip = ctypes.POINTER(ctypes.c_int)
+        lib().orc_cost_list_fp64(self.ptr, n, xs.ctypes.data_as(ip), ys.ctypes.data_as(ip),
+                                 _p(depth, _f32p), _p(normal, _f32p), _p(cost, _f64p),
+                                 _p(flags, _u32p), threads or os.cpu_count())
+        return cost, flags
+
     def init_fp32(self, seed):
         depth = np.zeros((self.H, self.W), dtype=np.float32)
         normal = np.zeros((self.H, self.W, 3), dtype=np.float32)
@@ -237,3 +287,58 @@ def geom_filter_fp32(depth, normal, confidence, Kinv, R, t, srcs, depth_tol=0.12
     lib().orc_geom_filter_fp32(W, H, _p(depth, _f32p), _p(normal, _f32p), _p(confidence, _f32p),
                                _p(ki, _f32p), _p(Rr, _f32p), _p(tt, _f32p), arr, len(srcs),
                                float(depth_tol), float(normal_tol_cos), int(min_consistent))
+
+
+class SfM:
+    """Cameras (dicts with K, R, t, optional registered) and points
+    ((position, image_ids) pairs) packed for the restatement."""
+
+    def __init__(self, cameras, points):
+        self.cams = (OrcCamera * max(1, len(cameras)))()
+        for i, c in enumerate(cameras):
+            cc = self.cams[i]
+            cc.image_id = i
+            cc.is_registered = int(c.get("registered", True))
+            for name in ("K", "R", "t"):
+                for j, v in enumerate(np.asarray(c[name], np.float64).reshape(-1)):
+                    getattr(cc, name)[j] = v
+        self.n_cams = len(cameras)
+        self.pts = (OrcPoint * max(1, len(points)))()
+        self._ids = []
+        for k, (pos, ids) in enumerate(points):
+            ids = np.ascontiguousarray(ids, dtype=np.int32)
+            self._ids.append(ids)
+            for j in range(3):
+                self.pts[k].position[j] = float(pos[j])
+            self.pts[k].image_ids = ids.ctypes.data_as(ctypes.POINTER(ctypes.c_int))
+            self.pts[k].num_image_ids = len(ids)
+        self.n_pts = len(points)
+
+    def select_source_views(self, num_sources=5):
+        out = (OrcViewParams * max(1, self.n_cams))()
+        n = lib().orc_select_source_views(self.cams, self.n_cams, self.pts, self.n_pts,
+                                          int(num_sources), out)
+        return [{"ref_image_id": out[i].ref_image_id,
+                 "sources": [(out[i].src_id[s], out[i].score[s], out[i].baseline_deg[s])
+                             for s in range(out[i].num_src)],
+                 "depth_min": out[i].depth_min, "depth_max": out[i].depth_max}
+                for i in range(n)]
+
+    def seeds(self, ref_id, dmin, dmax):
+        """[(x, y, depth, (nx, ny, nz))] in point order."""
+        out = (OrcSeed * max(1, self.n_pts))()
+        n = lib().orc_project_seeds(ctypes.byref(self.cams[ref_id]), self.pts, self.n_pts,
+                                    float(dmin), float(dmax), out)
+        return [(out[i].x, out[i].y, out[i].depth, tuple(out[i].normal)) for i in range(n)]
+
+
+def apply_seeds(depth, normal, seeds, radius=50):
+    """Paint seeds [(x, y, depth, n)] in order over depth (H,W) / normal (H,W,3), in place."""
+    H, W = depth.shape
+    arr = (OrcSeed * max(1, len(seeds)))()
+    for i, (x, y, d, n) in enumerate(seeds):
+        arr[i].x, arr[i].y, arr[i].depth = int(x), int(y), float(d)
+        for j in range(3):
+            arr[i].normal[j] = float(n[j])
+    assert depth.dtype == np.float32 and normal.dtype == np.float32
+    lib().orc_apply_seeds(W, H, arr, len(seeds), int(radius), _p(depth, _f32p), _p(normal, _f32p))

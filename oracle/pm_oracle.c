@@ -287,6 +287,38 @@ void orc_cost_map_fp64(const orc_view* v, const float* depth, const float* norma
     run_rows(costmap_row, &c, v->H, threads);
 }
 
+typedef struct {
+    const orc_view* v;
+    int n;
+    const int* xs;
+    const int* ys;
+    const float* depth;
+    const float* normal;
+    double* cost;
+    uint32_t* flags;
+} costlist_ctx;
+
+/* "row" r of the work split handles list entries r, r + rows, ... */
+#define LIST_ROWS 64
+static void costlist_row(void* p, int r)
+{
+    costlist_ctx* c = (costlist_ctx*)p;
+    for (int i = r; i < c->n; i += LIST_ROWS) {
+        uint32_t f = 0;
+        c->cost[i] = orc_cost_fp64(c->v, c->xs[i], c->ys[i], c->depth[i], c->normal[3 * i],
+                                   c->normal[3 * i + 1], c->normal[3 * i + 2], &f, NULL);
+        if (c->flags) c->flags[i] = f;
+    }
+}
+
+void orc_cost_list_fp64(const orc_view* v, int n, const int* xs, const int* ys,
+                        const float* depth, const float* normal, double* cost, uint32_t* flags,
+                        int threads)
+{
+    costlist_ctx c = {v, n, xs, ys, depth, normal, cost, flags};
+    run_rows(costlist_row, &c, LIST_ROWS, threads);
+}
+
 /* ------------------------------------------------------------------------ */
 /* FP32 cost, reference operation order (patchmatch_mvs.cu:222-343)          */
 /* ------------------------------------------------------------------------ */
@@ -674,4 +706,196 @@ void orc_geom_filter_fp32(int W, int H, float* depth, const float* normal, float
             }
             if (consistent < min_consistent) { depth[idx] = 0.0f; confidence[idx] = 0.0f; }
         }
+}
+
+/* ==========================================================================
+ * Host view set-up.  selectSourceViews and its helpers
+ * (src/dense_recon/patchmatch_mvs.cu:41-157) and the seeding block of the
+ * driver (:1111-1177), restated literally: shared points by scanning every
+ * point for every camera pair, the depth range from sorted visible depths.
+ * ======================================================================== */
+static int obs(const orc_point* p, int id)
+{
+    for (int k = 0; k < p->num_image_ids; ++k)
+        if (p->image_ids[k] == id) return 1;
+    return 0;
+}
+
+/* countSharedPoints (:41-52) */
+static int count_shared(const orc_point* pts, int n, int a, int b)
+{
+    int c = 0;
+    for (int k = 0; k < n; ++k)
+        if (obs(&pts[k], a) && obs(&pts[k], b)) c++;
+    return c;
+}
+
+static void cam_centre(const orc_camera* c, double C[3])
+{
+    for (int i = 0; i < 3; ++i)
+        C[i] = -(c->R[i] * c->t[0] + c->R[3 + i] * c->t[1] + c->R[6 + i] * c->t[2]);
+}
+
+static void cam_point(const orc_camera* c, const double X[3], double p[3])
+{
+    for (int i = 0; i < 3; ++i)
+        p[i] = (c->R[3 * i] * X[0] + c->R[3 * i + 1] * X[1] + c->R[3 * i + 2] * X[2]) + c->t[i];
+}
+
+static void unit3(double v[3])
+{
+    const double n = sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    if (n > 0.0)
+        for (int i = 0; i < 3; ++i) v[i] /= n;
+}
+
+/* computeBaselineAngle (:55-63) */
+static float baseline_angle(const orc_camera* a, const orc_camera* b, const double cen[3])
+{
+    double ca[3], cb[3], va[3], vb[3];
+    cam_centre(a, ca);
+    cam_centre(b, cb);
+    for (int i = 0; i < 3; ++i) { va[i] = cen[i] - ca[i]; vb[i] = cen[i] - cb[i]; }
+    unit3(va);
+    unit3(vb);
+    double c = va[0] * vb[0] + va[1] * vb[1] + va[2] * vb[2];
+    c = c < -1.0 ? -1.0 : (c > 1.0 ? 1.0 : c);
+    return (float)(acos(c) * 180.0 / 3.14159265358979323846);
+}
+
+/* baselineScore (:66-72): BASELINE_MIN_DEG 5, BASELINE_MAX_DEG 30 (:19-20) */
+static float baseline_score(float deg)
+{
+    if (deg < 5.0f || deg > 30.0f * 3.0f) return 0.0f;
+    const float c = deg < 5.0f ? 5.0f : (deg > 30.0f ? 30.0f : deg);
+    return sinf(c * (float)3.14159265358979323846 / 180.0f);
+}
+
+static int cmp_float(const void* a, const void* b)
+{
+    const float x = *(const float*)a, y = *(const float*)b;
+    return x < y ? -1 : (x > y ? 1 : 0);
+}
+
+int orc_select_source_views(const orc_camera* cams, int num_cams, const orc_point* pts,
+                            int num_pts, int num_sources, orc_view_params* out)
+{
+    double cen[3] = {0.0, 0.0, 0.0};
+    if (num_pts > 0) {
+        for (int k = 0; k < num_pts; ++k)
+            for (int i = 0; i < 3; ++i) cen[i] += pts[k].position[i];
+        for (int i = 0; i < 3; ++i) cen[i] /= (double)num_pts;
+    }
+    int n_out = 0;
+    int* cid = (int*)malloc(sizeof(int) * (num_cams > 0 ? num_cams : 1));
+    float* csc = (float*)malloc(sizeof(float) * (num_cams > 0 ? num_cams : 1));
+    float* cdeg = (float*)malloc(sizeof(float) * (num_cams > 0 ? num_cams : 1));
+    float* dep = (float*)malloc(sizeof(float) * (num_pts > 0 ? num_pts : 1));
+    for (int r = 0; r < num_cams; ++r) {
+        if (!cams[r].is_registered) continue;
+        const int ref = cams[r].image_id;
+        orc_view_params* vp = &out[n_out++];
+        memset(vp, 0, sizeof(*vp));
+        vp->ref_image_id = ref;
+        int nc = 0;
+        for (int s = 0; s < num_cams; ++s) {
+            if (!cams[s].is_registered) continue;
+            const int src = cams[s].image_id;
+            if (src == ref) continue;
+            const float ang = baseline_angle(&cams[ref], &cams[src], cen);
+            const float bs = baseline_score(ang);
+            if (bs <= 0.0f) continue;
+            const int sh = count_shared(pts, num_pts, ref, src);
+            if (sh == 0) continue;
+            /* insertion keeping descending score, equal scores in arrival order */
+            const float sc = (float)sh * bs;
+            int pos = nc;
+            while (pos > 0 && csc[pos - 1] < sc) {
+                cid[pos] = cid[pos - 1];
+                csc[pos] = csc[pos - 1];
+                cdeg[pos] = cdeg[pos - 1];
+                --pos;
+            }
+            cid[pos] = src;
+            csc[pos] = sc;
+            cdeg[pos] = ang;
+            ++nc;
+        }
+        vp->num_src = nc < num_sources ? nc : num_sources;
+        for (int k = 0; k < vp->num_src; ++k) {
+            vp->src_id[k] = cid[k];
+            vp->score[k] = csc[k];
+            vp->baseline_deg[k] = cdeg[k];
+        }
+        int nd = 0;
+        for (int k = 0; k < num_pts; ++k) {
+            if (!obs(&pts[k], ref)) continue;
+            double p[3];
+            cam_point(&cams[ref], pts[k].position, p);
+            if (p[2] > 0.0) dep[nd++] = (float)p[2];
+        }
+        if (nd >= 2) {
+            qsort(dep, (size_t)nd, sizeof(float), cmp_float);
+            const size_t i5 = (size_t)(nd * 0.05);
+            size_t i95 = (size_t)(nd * 0.95);
+            if (i95 > (size_t)nd - 1) i95 = (size_t)nd - 1;
+            vp->depth_min = dep[i5] * 0.5f;
+            vp->depth_max = dep[i95] * 1.5f;
+        } else {
+            double p[3];
+            cam_point(&cams[ref], cen, p);
+            const float d = (float)p[2];
+            vp->depth_min = d * 0.3f;
+            vp->depth_max = d * 3.0f;
+        }
+    }
+    free(cid);
+    free(csc);
+    free(cdeg);
+    free(dep);
+    return n_out;
+}
+
+int orc_project_seeds(const orc_camera* cam, const orc_point* pts, int num_pts, float dmin,
+                      float dmax, orc_seed* out)
+{
+    double C[3];
+    cam_centre(cam, C);
+    int n = 0;
+    for (int k = 0; k < num_pts; ++k) {
+        if (!obs(&pts[k], cam->image_id)) continue;
+        double p[3], q[3];
+        cam_point(cam, pts[k].position, p);
+        if (p[2] <= 0.0) continue;
+        for (int i = 0; i < 3; ++i)
+            q[i] = cam->K[3 * i] * p[0] + cam->K[3 * i + 1] * p[1] + cam->K[3 * i + 2] * p[2];
+        const int px = (int)round(q[0] / q[2]), py = (int)round(q[1] / q[2]);
+        const float z = (float)p[2];
+        if (z < dmin || z > dmax) continue;
+        double tc[3] = {C[0] - pts[k].position[0], C[1] - pts[k].position[1],
+                        C[2] - pts[k].position[2]};
+        unit3(tc);
+        orc_seed* s = &out[n++];
+        s->x = px;
+        s->y = py;
+        s->depth = z;
+        for (int i = 0; i < 3; ++i)
+            s->normal[i] = (float)(cam->R[3 * i] * tc[0] + cam->R[3 * i + 1] * tc[1] +
+                                   cam->R[3 * i + 2] * tc[2]);
+    }
+    return n;
+}
+
+void orc_apply_seeds(int W, int H, const orc_seed* seeds, int n, int radius, float* depth,
+                     float* normal)
+{
+    for (int k = 0; k < n; ++k)
+        for (int dy = -radius; dy <= radius; ++dy)
+            for (int dx = -radius; dx <= radius; ++dx) {
+                const int sx = seeds[k].x + dx, sy = seeds[k].y + dy;
+                if (sx < 0 || sx >= W || sy < 0 || sy >= H) continue;
+                const size_t q = (size_t)sy * W + sx;
+                depth[q] = seeds[k].depth;
+                for (int i = 0; i < 3; ++i) normal[3 * q + i] = seeds[k].normal[i];
+            }
 }

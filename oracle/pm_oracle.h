@@ -98,6 +98,11 @@ double orc_cost_fp64(const orc_view* v, int x, int y, float d, float nx, float n
 void orc_cost_map_fp64(const orc_view* v, const float* depth, const float* normal,
                        double* cost, uint32_t* flags, int threads);
 
+/* FP64 cost at n listed pixels (xs, ys) with hypotheses depth[i], normal[3i..]. */
+void orc_cost_list_fp64(const orc_view* v, int n, const int* xs, const int* ys,
+                        const float* depth, const float* normal, double* cost, uint32_t* flags,
+                        int threads);
+
 /* Same cost in single precision, reference operation order. */
 float orc_cost_fp32(const orc_view* v, int x, int y, float d, float nx, float ny,
                     float nz);
@@ -138,6 +143,40 @@ void orc_geom_filter_fp32(int W, int H, float* depth, const float* normal,
                           float* confidence, const float Kinv[9], const float R[9],
                           const float t[3], const orc_geom_src* srcs, int num_src,
                           float depth_tol, float normal_tol_cos, int min_consistent);
+
+/* ---- host view set-up (patchmatch_mvs.cu:41-157, 1111-1177) ---------------- */
+typedef struct {
+    int image_id, is_registered;
+    double K[9], R[9], t[3];
+} orc_camera;
+typedef struct {
+    double position[3];
+    const int* image_ids;
+    int num_image_ids;
+} orc_point;
+typedef struct {
+    int ref_image_id, num_src;
+    int src_id[ORC_MAX_SRC];
+    float score[ORC_MAX_SRC], baseline_deg[ORC_MAX_SRC];
+    float depth_min, depth_max;
+} orc_view_params;
+
+/* selectSourceViews, literally (per-pair point scans, insertion by score with
+ * ties kept in camera order).  out: one entry per registered camera. */
+int orc_select_source_views(const orc_camera* cams, int num_cams, const orc_point* pts,
+                            int num_pts, int num_sources, orc_view_params* out);
+
+typedef struct {
+    int x, y;
+    float depth, normal[3];
+} orc_seed;
+/* Seeds of camera cam, in point order (.cu:1126-1155), all of them (no image
+ * test): returns the count written to out. */
+int orc_project_seeds(const orc_camera* cam, const orc_point* pts, int num_pts, float dmin,
+                      float dmax, orc_seed* out);
+/* Paint seeds over depth (H*W) / normal (H*W*3) in order (.cu:1157-1169). */
+void orc_apply_seeds(int W, int H, const orc_seed* seeds, int n, int radius, float* depth,
+                     float* normal);
 
 #ifdef __cplusplus
 }

@@ -11,6 +11,7 @@ import math
 import os
 import re
 import struct
+import subprocess
 
 import numpy as np
 import pytest
@@ -175,3 +176,22 @@ def test_null_context_arguments(pkg):
     assert lib.hp3d_pm_destroy(None) == pkg.ERR_INVALID_ARG
     assert lib.hp3d_pm_run_view(None, None, None) == pkg.ERR_INVALID_ARG
     assert lib.hp3d_pfm_save(None, 1, 1, None) == pkg.ERR_INVALID_ARG
+
+
+CPP_TEST = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "bin",
+                        "test_patchmatch")
+
+
+def _cpp_test_binary():
+    if not os.path.exists(CPP_TEST):
+        subprocess.check_call(["make", "-s", "-C", os.path.dirname(os.path.dirname(CPP_TEST))])
+    return CPP_TEST
+
+
+def test_cpp_host_api_without_gpu(pkg):
+    """The C++ mirror of the reference API (selectSourceViews, PFM, legacy
+    stubs) through the reference's own test names, host-only part."""
+    r = subprocess.run([_cpp_test_binary(), "--cpu"], capture_output=True, text=True, timeout=300)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 test(s) failed" in r.stdout
