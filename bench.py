@@ -53,6 +53,11 @@ def parse():
     ap.add_argument("--duration", type=float, default=120.0, help="serving: arrival window (s)")
     ap.add_argument("--stride", type=int, default=None,
                     help="batched/serving window stride (default: 1 batched, 2 serving)")
+    ap.add_argument("--max-jobs", type=int, default=64, help="serving: concurrent job slots")
+    ap.add_argument("--variant", type=int, default=-1,
+                    help="batched/serving: step-kernel variant (-1: autotuned / default)")
+    ap.add_argument("--form-store", choices=["f32", "f16"], default="f16",
+                    help="batched/serving image storage")
     return ap.parse_args()
 
 
@@ -293,22 +298,29 @@ def form_batched(a, entry, pkg, scenes):
                                      stride=j.stride, iterations=j.iterations, seed=j.seed))
         return out
 
-    d16 = descs_for(pkg.STORE_F16)
+    fstore = pkg.STORE_F16 if a.form_store == "f16" else pkg.STORE_F32
+    d16 = descs_for(fstore)
     outs = [_device_outputs(ctx, j.shape) for j in jobs]
-    best, times = ctx.autotune(d16)
-    ctx.batch_run_device(d16, outs)                       # warm-up (captures the graph)
+    if a.variant >= 0:
+        best, times = a.variant, np.full(pkg.NUM_VARIANTS, -1.0)
+    else:
+        best, times = ctx.autotune(d16)
+    ctx.batch_run_device(d16, outs, variant=best)         # warm-up (captures the graph)
     reps = 2
     t0 = time.perf_counter()
     for _ in range(reps):
-        ctx.batch_run_device(d16, outs)
+        ctx.batch_run_device(d16, outs, variant=best)
     batched_s = (time.perf_counter() - t0) / reps
-    # plain kernel, FP32 storage, one view after the other
-    d32 = descs_for(pkg.STORE_F32)
-    ctx.run_view_device(d32[0], *outs[0])
-    t0 = time.perf_counter()
-    for d, o in zip(d32, outs):
-        ctx.run_view_device(d, *o)
-    plain_s = time.perf_counter() - t0
+    # plain kernel one view after the other: FP32 storage, and the batch's storage
+    def serial(descs):
+        ctx.run_view_device(descs[0], *outs[0])
+        t0 = time.perf_counter()
+        for d, o in zip(descs, outs):
+            ctx.run_view_device(d, *o)
+        return time.perf_counter() - t0
+    plain_same_s = serial(d16)
+    d32 = descs_for(pkg.STORE_F32) if fstore != pkg.STORE_F32 else d16
+    plain_s = serial(d32)
     cpu = None
     if not a.no_cpu:
         rate = _cpu_tap_rate(entry, a.cpu_threads)
@@ -320,7 +332,7 @@ def form_batched(a, entry, pkg, scenes):
     pkg.lib().hp3d_pm_variant_info(best, *[ctypes.byref(c) for c in ci])
     value = mpix / batched_s
     out = {"form": "batched", "metric": "PatchMatch Mpixel-iters/s", "value": round(value, 2),
-           "unit": "Mpixel-iters/s", "dtype": "fp16 storage / fp32 accumulation",
+           "unit": "Mpixel-iters/s", "dtype": f"{a.form_store} storage / fp32 accumulation",
            "data": "synthetic", "config": {
                "workload": f"batched form: {a.views} views, resolutions over "
                            "{640x480,1280x960,1600x1200,2400x1800,4000x3000}, 3-8 sources, "
@@ -330,6 +342,7 @@ def form_batched(a, entry, pkg, scenes):
            "ms_per_batch": round(batched_s * 1e3, 2),
            "plain_serial_value": round(mpix / plain_s, 2),
            "speedup_vs_plain_serial": round(plain_s / batched_s, 3),
+           "plain_serial_same_storage_value": round(mpix / plain_same_s, 2),
            "autotune": {"variant": best, "tile": [ci[0].value, ci[1].value],
                         "waves_per_simd_hint": ci[2].value,
                         "times_ms": [round(float(t), 3) for t in times]},
@@ -354,8 +367,9 @@ def form_serving(a, entry, pkg, scenes):
     picks = rng.integers(0, len(pool), n_arr)
     iters = rng.integers(4, 13, n_arr)
     offered = sum(pool[p].shape[0] * pool[p].shape[1] * it for p, it in zip(picks, iters)) / 1e6 / a.duration
-    srv = pkg.Server(hbm_budget=96 << 30, max_jobs=64, radius=5, stride=stride,
-                     store=pkg.STORE_F16)
+    srv = pkg.Server(hbm_budget=96 << 30, max_jobs=a.max_jobs, radius=5, stride=stride,
+                     store=pkg.STORE_F16 if a.form_store == "f16" else pkg.STORE_F32,
+                     variant=a.variant)
     for w in range(2):                                   # warm-up
         srv.wait(srv.submit_job(pool[w], iterations=4), timeout=120)
     lat, done_mpix, t_first, t_last = [], 0.0, None, None
@@ -428,11 +442,12 @@ def form_serving(a, entry, pkg, scenes):
     lat = np.array(lat) * 1e3
     value = done_mpix / wall
     out = {"form": "serving", "metric": "PatchMatch Mpixel-iters/s; p50/p99 job latency",
-           "value": round(value, 2), "unit": "Mpixel-iters/s",
+           "value": round(value, 2), "unit": "Mpixel-iters/s", "max_jobs": a.max_jobs,
+           "variant": a.variant,
            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
                           "p99": round(float(np.percentile(lat, 99)), 2),
                           "mean": round(float(lat.mean()), 2)},
-           "dtype": "fp16 storage / fp32 accumulation", "data": "synthetic",
+           "dtype": f"{a.form_store} storage / fp32 accumulation", "data": "synthetic",
            "config": {"workload": f"serving form: Poisson {a.rate}/s for {a.duration}s, 1 ref + "
                                   "3-8 src from the batched resolution mix, 4-12 iterations, "
                                   f"11x11 stride {stride}", "pool_jobs": len(pool)},

@@ -58,7 +58,7 @@ class ViewDesc(ctypes.Structure):
         ("top_k", ctypes.c_int), ("seed", ctypes.c_uint64),
         ("init_depth", _f32p), ("init_normal", _f32p),
         ("seeds", ctypes.POINTER(SeedPoint)), ("num_seeds", ctypes.c_int),
-        ("seed_radius", ctypes.c_int),
+        ("seed_radius", ctypes.c_int), ("first_iteration", ctypes.c_int),
     ]
 
 
@@ -221,6 +221,11 @@ SIGNATURES = {
     "hp3d_pfm_save": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int, _f32p]),
     "hp3d_pfm_load": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
                                      ctypes.POINTER(ctypes.c_int), _f32p, ctypes.c_size_t]),
+    "hp3d_pfm_save_ex": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_int, ctypes.c_int,
+                                        ctypes.c_int, _f32p]),
+    "hp3d_pfm_load_ex": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int), _f32p, ctypes.c_size_t]),
     "hp3d_scene_render": (ctypes.c_int, [ctypes.POINTER(Scene), ctypes.POINTER(Camera),
                                          ctypes.c_double, ctypes.c_uint64, _u8p, _f32p, _f32p,
                                          ctypes.c_int]),
@@ -284,6 +289,25 @@ def pfm_save(path, depth):
     depth = np.ascontiguousarray(depth, dtype=np.float32)
     H, W = depth.shape
     _check(lib().hp3d_pfm_save(os.fsencode(path), W, H, _p(depth, _f32p)))
+
+
+def pfm_save_ex(path, data):
+    """(H, W) -> "Pf", (H, W, 3) -> "PF"."""
+    data = np.ascontiguousarray(data, dtype=np.float32)
+    H, W = data.shape[:2]
+    C = 1 if data.ndim == 2 else data.shape[2]
+    _check(lib().hp3d_pfm_save_ex(os.fsencode(path), W, H, C, _p(data, _f32p)))
+
+
+def pfm_load_ex(path):
+    W, H, C = ctypes.c_int(0), ctypes.c_int(0), ctypes.c_int(0)
+    _check(lib().hp3d_pfm_load_ex(os.fsencode(path), ctypes.byref(W), ctypes.byref(H),
+                                  ctypes.byref(C), None, 0))
+    shape = (H.value, W.value) if C.value == 1 else (H.value, W.value, C.value)
+    out = np.zeros(shape, dtype=np.float32)
+    _check(lib().hp3d_pfm_load_ex(os.fsencode(path), ctypes.byref(W), ctypes.byref(H),
+                                  ctypes.byref(C), _p(out, _f32p), out.size))
+    return out
 
 
 def pfm_load(path):
@@ -563,7 +587,8 @@ class SfM:
 
 
 def make_desc(ref, srcs, K_ref_inv, A, b, depth_min, depth_max, radius=5, stride=1,
-              iterations=0, top_k=3, seed=42, init=None, seeds=None, seed_radius=50):
+              iterations=0, top_k=3, seed=42, init=None, seeds=None, seed_radius=50,
+              first_iteration=0):
     """Build a ViewDesc.  ref/srcs are image handles; A, b lists of FP64 terms.
 
     init: optional (depth (H,W), normal (H,W,3)) float32 host arrays replacing
@@ -587,6 +612,7 @@ def make_desc(ref, srcs, K_ref_inv, A, b, depth_min, depth_max, radius=5, stride
     d.depth_min, d.depth_max = float(depth_min), float(depth_max)
     d.radius, d.stride, d.iterations, d.top_k = int(radius), int(stride), int(iterations), int(top_k)
     d.seed = int(seed)
+    d.first_iteration = int(first_iteration)
     if init is not None:
         idp = np.ascontiguousarray(init[0], dtype=np.float32)
         inn = np.ascontiguousarray(init[1], dtype=np.float32)

@@ -125,9 +125,9 @@ int validate_batch(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
     for (int j = 0; j < n; ++j) {
         int rc = validate_desc(ctx, &descs[j], &refs[j], srcs[j].data());
         if (rc) return ctx->fail(rc, "batch view %d: %s", j, ctx->last_error.c_str());
-        if (descs[j].init_depth || descs[j].num_seeds > 0)
-            return ctx->fail(HP3D_ERR_INVALID_ARG, "batch view %d: caller hypotheses and SfM "
-                             "seeds are only supported by hp3d_pm_run_view", j);
+        if (descs[j].init_depth || descs[j].num_seeds > 0 || descs[j].first_iteration != 0)
+            return ctx->fail(HP3D_ERR_INVALID_ARG, "batch view %d: caller hypotheses, SfM seeds "
+                             "and resumed runs are only supported by hp3d_pm_run_view", j);
         if (descs[j].radius != descs[0].radius || descs[j].stride != descs[0].stride ||
             refs[j]->store != refs[0]->store)
             return ctx->fail(HP3D_ERR_INVALID_ARG, "batch view %d: radius, stride and storage "

@@ -91,6 +91,9 @@ int hp3d::validate_desc(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image**
         return ctx->fail(HP3D_ERR_INVALID_ARG, "top_k %d outside 1..%d", d->top_k, kMaxTopK);
     if (d->iterations < 0)
         return ctx->fail(HP3D_ERR_INVALID_ARG, "iterations %d < 0", d->iterations);
+    if (d->first_iteration < 0 || d->first_iteration > (1 << 20))
+        return ctx->fail(HP3D_ERR_INVALID_ARG, "first_iteration %d outside 0..2^20",
+                         d->first_iteration);
     if (!(d->depth_min < d->depth_max) || !std::isfinite(d->depth_min) ||
         !std::isfinite(d->depth_max))
         return ctx->fail(HP3D_ERR_INVALID_ARG, "depth range [%g, %g] is empty", d->depth_min,

@@ -65,10 +65,23 @@ __global__ __launch_bounds__(TX * TY) void pm_cost_kernel(DevView v, int parity,
     cost_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, threadIdx.x % TX, threadIdx.x / TX, parity);
 }
 
+// Build-time tuning knobs of the per-view kernels (tile shape, minimum waves
+// per SIMD the register allocation must allow); the defaults are the fastest
+// measured on the 1600x1200 workload.
+#ifndef HP3D_PLAIN_TX
+#define HP3D_PLAIN_TX 32
+#endif
+#ifndef HP3D_PLAIN_TY
+#define HP3D_PLAIN_TY 8
+#endif
+#ifndef HP3D_PLAIN_WPE
+#define HP3D_PLAIN_WPE 1
+#endif
+
 template <int NT, int ST, int STORE, int TX, int TY>
-__global__ __launch_bounds__(TX * TY) void pm_halfpass_kernel(DevView v, int iteration, int parity,
-                                                              uint64_t pass_seed, int tiles_x,
-                                                              int tiles)
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(HP3D_PLAIN_WPE, 8)))
+void pm_halfpass_kernel(DevView v, int iteration, int parity, uint64_t pass_seed, int tiles_x,
+                        int tiles)
 {
     constexpr int R = (NT - 1) * ST / 2;
     using G = TileGeom<R, STORE, TX, TY>;
@@ -175,9 +188,8 @@ hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_
     return hipGetLastError();
 }
 
-// 32 compacted columns (64 pixels) x 8 rows per workgroup: the fastest
-// shape of the step-kernel sweep on the 1600x1200 workload.
-constexpr int kTX = 32, kTY = 8;
+// 32 compacted columns (64 pixels) x 8 rows per workgroup by default.
+constexpr int kTX = HP3D_PLAIN_TX, kTY = HP3D_PLAIN_TY;
 
 template <int NT, int ST, int STORE>
 static void launch_pass_t(const DevView& v, int mode, int iteration, int parity,

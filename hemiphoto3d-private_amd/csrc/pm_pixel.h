@@ -148,16 +148,40 @@ struct TapSums {
     int cnt;
 };
 
+// Sums of the reference window over every tap, in tap order: they do not
+// depend on the candidate or the source, so a window whose taps are all
+// valid takes them from here instead of accumulating them per source.
+struct WinRef {
+    float sr, srr;
+};
+template <int NT, int ST>
+__device__ __forceinline__ WinRef window_ref_sums(const float* __restrict__ lwin, int lp)
+{
+    float sr = 0.f, srr = 0.f;
+#pragma unroll 1
+    for (int j = 0; j < NT; ++j) {
+        const float* lrow = lwin + j * ST * lp;
+#pragma unroll
+        for (int i = 0; i < NT; ++i) {
+            const float r = lrow[i * ST];
+            sr += r;
+            srr = fmaf(r, r, srr);
+        }
+    }
+    return WinRef{sr, srr};
+}
+
 // The window's taps for one candidate and source (patchmatch_mvs.cu:273-307).
 // MASKED = false is used only when every tap is known to be inside the image;
-// both variants perform the same floating-point operations for valid taps, so
-// a window's sums do not depend on which variant its wave ran.
+// it takes the reference sums from wr, which window_ref_sums forms with the
+// same additions in the same order as the masked loop does when every tap is
+// valid, so a window's sums do not depend on which variant its wave ran.
 template <int NT, int ST, int STORE, bool MASKED>
 __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
                                            const float* __restrict__ lwin, int lp, float cx,
                                            float cy, float cz, float h00, float h01, float h10,
                                            float h11, float h20, float h21, unsigned umax,
-                                           unsigned vmax, TapSums& m)
+                                           unsigned vmax, const WinRef& wr, TapSums& m)
 {
     constexpr int R = (NT - 1) * ST / 2;
     float sr = 0.f, ss = 0.f, srr = 0.f, sss = 0.f, srs = 0.f;
@@ -189,14 +213,18 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
                 r = ok ? r : 0.0f;
                 cnt += ok ? 1 : 0;
             }
-            sr += r;
+            if (MASKED) {
+                sr += r;
+                srr = fmaf(r, r, srr);
+            }
             ss += sval;
-            srr = fmaf(r, r, srr);
             sss = fmaf(sval, sval, sss);
             srs = fmaf(r, sval, srs);
         }
     }
-    m.sr = sr; m.ss = ss; m.srr = srr; m.sss = sss; m.srs = srs;
+    m.sr = MASKED ? sr : wr.sr;
+    m.srr = MASKED ? srr : wr.srr;
+    m.ss = ss; m.sss = sss; m.srs = srs;
     m.cnt = MASKED ? cnt : NT * NT;
 }
 
@@ -213,8 +241,9 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
 // ---------------------------------------------------------------------------
 template <int NT, int ST, int STORE>
 __device__ __forceinline__ float cand_cost(const DevView& v, const float* __restrict__ lwin,
-                                           int lp, int x, int y, double rx, double ry, double rz,
-                                           float d, float nx, float ny, float nz)
+                                           int lp, const WinRef& wr, int x, int y, double rx,
+                                           double ry, double rz, float d, float nx, float ny,
+                                           float nz)
 {
     constexpr int R = (NT - 1) * ST / 2;
     constexpr int kMinCount = (NT * NT) / 2;
@@ -271,10 +300,10 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
         TapSums m;
         if (__all(inside))
             tap_window<NT, ST, STORE, false>(img, lwin, lp, cx, cy, cz, h00, h01, h10, h11, h20,
-                                             h21, umax, vmax, m);
+                                             h21, umax, vmax, wr, m);
         else
             tap_window<NT, ST, STORE, true>(img, lwin, lp, cx, cy, cz, h00, h01, h10, h11, h20,
-                                            h21, umax, vmax, m);
+                                            h21, umax, vmax, wr, m);
         const int cnt = m.cnt;
         const float sr = m.sr, ss = m.ss, srr = m.srr, sss = m.sss, srs = m.srs;
         float c = 2.0f;
@@ -368,8 +397,9 @@ __device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, 
     ray_fp64(v.kinv_d, x, y, rx, ry, rz);
     const int p = y * v.Wc + i;
     const float4 h = v.hyp[parity][p];
+    const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
     v.cost[parity][p] =
-        cand_cost<NT, ST, STORE>(v, lwin, G::LP, x, y, rx, ry, rz, h.x, h.y, h.z, h.w);
+        cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x, y, rx, ry, rz, h.x, h.y, h.z, h.w);
     return true;
 }
 
@@ -401,6 +431,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
     const int p = y * Wc + i;
     float4 best = own[p];
     float bc = v.cost[parity][p];
+    const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
 
     // neighbour positions in the other plane: L = i+o-1, R = i+o (row y),
     // U, D = i (rows y-1, y+1)
@@ -411,8 +442,8 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
         const int q = k == 0 ? p + o - 1 : k == 1 ? p + o : k == 2 ? p - Wc : p + Wc;
         const float4 h = oth[q];
         if (h.x < dmin || h.x > dmax) continue;
-        const float c = cand_cost<NT, ST, STORE>(v, lwin, G::LP, x, y, rx, ry, rz, h.x, h.y, h.z,
-                                                 h.w);
+        const float c = cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x, y, rx, ry, rz, h.x, h.y,
+                                                 h.z, h.w);
         if (c < bc) { bc = c; best = h; }
     }
 
@@ -435,8 +466,8 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
         const float nlen = sqrtf(cnx * cnx + cny * cny + cnz * cnz);
         if (nlen > 1e-12f) { cnx = cnx / nlen; cny = cny / nlen; cnz = cnz / nlen; }
         if (cnx * ux + cny * uy + cnz * uz > 0.0f) { cnx = -cnx; cny = -cny; cnz = -cnz; }
-        const float c = cand_cost<NT, ST, STORE>(v, lwin, G::LP, x, y, rx, ry, rz, cd, cnx, cny,
-                                                 cnz);
+        const float c = cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x, y, rx, ry, rz, cd, cnx,
+                                                 cny, cnz);
         if (c < bc) { bc = c; best = make_float4(cd, cnx, cny, cnz); }
         depth_range *= 0.5f;
         normal_range *= 0.5f;

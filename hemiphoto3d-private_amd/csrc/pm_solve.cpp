@@ -137,7 +137,7 @@ int solve_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
     if (e != hipSuccess) return ctx->hip_fail(e, "cost launch");
     ctx->mark(kPhaseCost);
     int n_prop = 0;
-    for (int it = 0; it < d->iterations; ++it)
+    for (int it = d->first_iteration; it < d->first_iteration + d->iterations; ++it)
         for (int parity = 0; parity < 2; ++parity) {
             e = launch_pass(v, ref->store, d->radius, d->stride, kPassPropagate, it, parity,
                             pass_seed(d->seed, it, parity), ctx->stream);

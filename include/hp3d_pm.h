@@ -88,6 +88,11 @@ typedef struct {
     const struct hp3d_seed_point* seeds; /* optional SfM seeds (hp3d_project_seed_points), */
     int num_seeds;                       /*   applied after the init, before the first cost */
     int seed_radius;                     /* half side of each seeded square; reference 50 */
+    int first_iteration;                 /* schedule index of the first iteration run: 0, or
+                                            k to resume a run saved after k iterations from
+                                            its depth and normals (init_depth/init_normal);
+                                            the result is then identical to an uninterrupted
+                                            run.  hp3d_pm_run_view only. */
 } hp3d_view_desc;
 
 typedef struct {
@@ -356,6 +361,14 @@ int hp3d_project_seed_points(const hp3d_sfm_camera* cam, const hp3d_sfm_point* p
  * bottom to top.  hp3d_pfm_load with data == NULL only reports the size. */
 int hp3d_pfm_save(const char* path, int width, int height, const float* depth);
 int hp3d_pfm_load(const char* path, int* width, int* height, float* data, size_t capacity);
+/* Any PFM: channels 1 ("Pf", e.g. depth or confidence) or 3 ("PF", e.g. the
+ * H*W*3 normal map), same row order.  With these a run's depth and normals
+ * can be saved and resumed (hp3d_view_desc.first_iteration); the reference
+ * saves depth only (.cu:1254-1257).  Load with data == NULL reports the size
+ * and channel count only; capacity is in floats. */
+int hp3d_pfm_save_ex(const char* path, int width, int height, int channels, const float* data);
+int hp3d_pfm_load_ex(const char* path, int* width, int* height, int* channels, float* data,
+                     size_t capacity);
 
 #ifdef __cplusplus
 }

@@ -111,6 +111,26 @@ def test_pfm_load_big_endian_and_errors(pkg, tmp_path):
         pkg.pfm_load(str(tmp_path / "missing.pfm"))
 
 
+def test_pfm_three_channel_normals(pkg, tmp_path):
+    """Normal maps as "PF" (three interleaved channels, rows bottom to top)."""
+    n = np.random.default_rng(4).normal(size=(4, 6, 3)).astype(np.float32)
+    p = str(tmp_path / "n.pfm")
+    pkg.pfm_save_ex(p, n)
+    raw = open(p, "rb").read()
+    header = b"PF\n6 4\n-1.0\n"
+    assert raw.startswith(header)
+    body = np.frombuffer(raw[len(header):], dtype="<f4").reshape(4, 6, 3)
+    assert np.array_equal(body[::-1], n)
+    assert np.array_equal(pkg.pfm_load_ex(p), n)
+    with pytest.raises(pkg.HP3DError):       # the depth loader takes "Pf" only
+        pkg.pfm_load(p)
+    d = np.random.default_rng(5).uniform(1, 2, size=(3, 2)).astype(np.float32)
+    q = str(tmp_path / "d.pfm")
+    pkg.pfm_save_ex(q, d)
+    assert open(q, "rb").read().startswith(b"Pf\n2 3\n-1.0\n")
+    assert np.array_equal(pkg.pfm_load(q), d) and np.array_equal(pkg.pfm_load_ex(q), d)
+
+
 def test_plane_scene_ground_truth_closed_form(scenes, c1):
     H, W = c1.shape
     y = np.arange(H)[:, None].astype(np.float64)

@@ -21,6 +21,11 @@ How each oracle function is pinned (reference = src/dense_recon/patchmatch_mvs.c
   orc_patchmatch_*    known answer: converges to the analytic plane depth;
                       PropagationReducesCost analogue (test_patchmatch.cpp:356-407).
   orc_geom_filter     consistent GT maps retained, inconsistent maps removed.
+  orc_select_source_views, orc_project_seeds, orc_apply_seeds
+                      tests/test_views.py: a camera ring with known baseline
+                      angles and scores, depth percentiles of points on the
+                      optical axis, the centroid fallback, projection of a
+                      point straight ahead, overwrite order of seeds.
 """
 import json
 import math
