@@ -135,6 +135,13 @@ class GeomView(ctypes.Structure):
                 ("src", ctypes.c_int * MAX_SRC)]
 
 
+class FuseView(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_int), ("height", ctypes.c_int), ("depth", _f32p),
+                ("normal", _f32p), ("confidence", _f32p), ("color", _u8p),
+                ("color_channels", ctypes.c_int), ("K", ctypes.c_double * 9),
+                ("R", ctypes.c_double * 9), ("t", ctypes.c_double * 3)]
+
+
 class Stats(ctypes.Structure):
     _fields_ = [("ms_total", ctypes.c_double), ("ms_init", ctypes.c_double),
                 ("ms_cost", ctypes.c_double), ("ms_propagate", ctypes.c_double),
@@ -191,6 +198,9 @@ SIGNATURES = {
     "hp3d_pm_geom_filter": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(GeomView), ctypes.c_int,
                                            ctypes.c_float, ctypes.c_float, ctypes.c_int,
                                            ctypes.POINTER(ctypes.c_int)]),
+    "hp3d_pm_fuse": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(FuseView), ctypes.c_int,
+                                    ctypes.c_float, _f32p, _f32p, _f32p, ctypes.c_size_t,
+                                    ctypes.POINTER(ctypes.c_size_t)]),
     "hp3d_server_create": (ctypes.c_int, [ctypes.POINTER(ServerOpts),
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "hp3d_server_submit": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(JobDesc),
@@ -496,6 +506,46 @@ class Context:
         _check(lib().hp3d_pm_geom_filter(self.h, arr, n, float(depth_tol), float(normal_tol_deg),
                                          int(min_consistent), ret), self.h)
         return list(ret)
+
+    def fuse(self, maps, min_confidence=0.0, with_normals=True, with_colors=True):
+        """Fuse depth maps into points.  maps: dicts with depth (H,W), optional
+        normal (H,W,3), confidence (H,W), color (H,W) or (H,W,3) uint8, and
+        K, R, t.  Returns (positions, normals, colors) as (N, 3) float32."""
+        n = len(maps)
+        arr = (FuseView * n)()
+        keep = []
+        for i, m in enumerate(maps):
+            g = arr[i]
+            d = np.ascontiguousarray(m["depth"], np.float32)
+            keep.append(d)
+            g.height, g.width = d.shape
+            g.depth = _p(d, _f32p)
+            for k, t in (("normal", np.float32), ("confidence", np.float32)):
+                if m.get(k) is not None:
+                    a = np.ascontiguousarray(m[k], t)
+                    keep.append(a)
+                    setattr(g, k, _p(a, _f32p))
+            if m.get("color") is not None:
+                c = np.ascontiguousarray(m["color"], np.uint8)
+                keep.append(c)
+                g.color = _p(c, _u8p)
+                g.color_channels = 1 if c.ndim == 2 else c.shape[2]
+            for name, cnt in (("K", 9), ("R", 9), ("t", 3)):
+                for j, val in enumerate(np.asarray(m[name], np.float64).reshape(cnt)):
+                    getattr(g, name)[j] = val
+        cnt = ctypes.c_size_t(0)
+        _check(lib().hp3d_pm_fuse(self.h, arr, n, float(min_confidence), None, None, None, 0,
+                                  ctypes.byref(cnt)), self.h)
+        N = cnt.value
+        pos = np.zeros((N, 3), np.float32)
+        nrm = np.zeros((N, 3), np.float32) if with_normals else None
+        col = np.zeros((N, 3), np.float32) if with_colors else None
+        if N:
+            _check(lib().hp3d_pm_fuse(self.h, arr, n, float(min_confidence), _p(pos, _f32p),
+                                      _p(nrm, _f32p) if with_normals else None,
+                                      _p(col, _f32p) if with_colors else None, N,
+                                      ctypes.byref(cnt)), self.h)
+        return pos, nrm, col
 
     def autotune(self, descs):
         n = len(descs)

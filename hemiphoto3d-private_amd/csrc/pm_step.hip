@@ -9,10 +9,14 @@
 // views can join between steps.
 //
 // Work decomposition: every view is cut into TX x TY compacted tiles; all
-// items' tiles form one logical range.  Workgroups are dispatched to the 8
-// XCDs round-robin (block b -> XCD b % 8), so block b takes logical tile
-// (b % 8) * (nblocks / 8) + b / 8: each XCD works through a contiguous run of
-// tiles, and tiles that read the same source rows share that XCD's L2.
+// items' tiles form one logical range, cut into chunks of kStepChunk
+// consecutive tiles.  Workgroups are dispatched to the 8 XCDs round-robin
+// (block b -> XCD b % 8); XCD x takes chunks x, x + 8, x + 16, ... in order.
+// Tiles of a chunk are neighbours in one view and read the same source rows
+// through that XCD's L2, while every XCD sees an even sample of the whole
+// step -- views differ in source count, START items cost a third of a
+// half-pass, and early passes cost more than late ones, so contiguous
+// eighths of the range would leave XCDs idle at the end of the step.
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
@@ -21,6 +25,8 @@
 #include "pm_step.h"
 
 namespace hp3d {
+
+constexpr int kStepChunk = 16;   // tiles per chunk of the XCD interleave
 
 template <int NT, int ST, int STORE, int TX, int TY, int WPE>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
@@ -31,9 +37,9 @@ void pm_step_kernel(const DevJob* __restrict__ jobs, const StepItem* __restrict_
     using G = TileGeom<R, STORE, TX, TY>;
     __shared__ float tile[G::LH * G::LP];
 
-    const int nb = gridDim.x;                       // a multiple of 8
-    const int b = blockIdx.x;
-    const int t = (b & 7) * (nb >> 3) + (b >> 3);
+    const int b = blockIdx.x;                       // gridDim.x: a multiple of 8 * kStepChunk
+    const int sidx = b >> 3;                        // position in this XCD's sequence
+    const int t = ((sidx / kStepChunk) * 8 + (b & 7)) * kStepChunk + sidx % kStepChunk;
     if (t >= total_tiles) return;
     int lo = 0, hi = n_items - 1;                   // last item with tile_begin <= t
     while (lo < hi) {
@@ -83,14 +89,18 @@ template <int NT, int ST, int STORE, int TX, int TY, int WPE>
 static hipError_t launch_t(const DevJob* jobs, const StepItem* items, int n_items, int total_tiles,
                            hipStream_t st)
 {
-    const int grid = ((total_tiles + 7) / 8) * 8;
+    constexpr int kGroup = 8 * kStepChunk;
+    const int grid = ((total_tiles + kGroup - 1) / kGroup) * kGroup;
     hipLaunchKernelGGL((pm_step_kernel<NT, ST, STORE, TX, TY, WPE>), dim3(grid), dim3(TX * TY), 0,
                        st, jobs, items, n_items, total_tiles);
     return hipGetLastError();
 }
 
+// 256-lane tiles of four shapes and two 128-lane tiles (finer granularity
+// for small views).  Variants that force more waves per SIMD than the
+// register budget allows spill to scratch and are not offered.
 const StepVariant kStepVariants[] = {
-    {16, 16, 1}, {32, 8, 1}, {8, 32, 1}, {64, 4, 1}, {16, 16, 5}, {32, 8, 5},
+    {16, 16, 1}, {32, 8, 1}, {8, 32, 1}, {64, 4, 1}, {32, 4, 1}, {16, 8, 1},
 };
 const int kNumStepVariants = (int)(sizeof(kStepVariants) / sizeof(kStepVariants[0]));
 
@@ -103,8 +113,8 @@ static hipError_t launch_tuned(int variant, const DevJob* jobs, const StepItem* 
         case 1: return launch_t<NT, ST, STORE, 32, 8, 1>(jobs, items, n_items, total_tiles, st);
         case 2: return launch_t<NT, ST, STORE, 8, 32, 1>(jobs, items, n_items, total_tiles, st);
         case 3: return launch_t<NT, ST, STORE, 64, 4, 1>(jobs, items, n_items, total_tiles, st);
-        case 4: return launch_t<NT, ST, STORE, 16, 16, 5>(jobs, items, n_items, total_tiles, st);
-        case 5: return launch_t<NT, ST, STORE, 32, 8, 5>(jobs, items, n_items, total_tiles, st);
+        case 4: return launch_t<NT, ST, STORE, 32, 4, 1>(jobs, items, n_items, total_tiles, st);
+        case 5: return launch_t<NT, ST, STORE, 16, 8, 1>(jobs, items, n_items, total_tiles, st);
         default: return hipErrorInvalidValue;
     }
 }

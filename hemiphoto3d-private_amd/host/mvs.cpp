@@ -291,6 +291,66 @@ std::vector<DepthMap> runDenseReconstruction(const std::vector<ImageData>&,
     return {};   // legacy stub in the reference as well
 }
 
+DensePointCloud fuseDepthMaps(const std::vector<DepthMap>& depth_maps,
+                              const std::vector<CameraPose>& poses,
+                              const std::vector<ImageData>& images, float min_confidence,
+                              int device)
+{
+    DensePointCloud pc;
+    std::vector<hp3d_fuse_view> fv;
+    std::vector<std::vector<uint8_t>> gray;
+    for (const DepthMap& dm : depth_maps) {
+        const size_t px = (size_t)dm.width * dm.height;
+        if (dm.image_id < 0 || dm.image_id >= (int)poses.size() || dm.depth.size() != px)
+            throw MVSError(HP3D_ERR_INVALID_ARG, "fuseDepthMaps: map without a pose or depth");
+        hp3d_fuse_view v{};
+        v.width = dm.width;
+        v.height = dm.height;
+        v.depth = dm.depth.data();
+        v.normal = dm.normal.size() == 3 * px ? dm.normal.data() : nullptr;
+        v.confidence = dm.confidence.size() == px ? dm.confidence.data() : nullptr;
+        if (dm.image_id < (int)images.size()) {
+            const ImageData& im = images[dm.image_id];
+            if (im.gray_u8.size() == px) {
+                v.color = im.gray_u8.data();
+            } else if (im.gray.size() == px) {
+                gray.emplace_back(px);
+                for (size_t p = 0; p < px; ++p)
+                    gray.back()[p] = (uint8_t)std::lround(
+                        std::min(1.0f, std::max(0.0f, im.gray[p])) * 255.0f);
+                v.color = gray.back().data();
+            }
+            v.color_channels = v.color ? 1 : 0;
+        }
+        const CameraPose& c = poses[dm.image_id];
+        for (int k = 0; k < 9; ++k) {
+            v.K[k] = c.K[k];
+            v.R[k] = c.R[k];
+        }
+        for (int k = 0; k < 3; ++k) v.t[k] = c.t[k];
+        fv.push_back(v);
+    }
+    if (fv.empty()) return pc;
+    hp3d_pm_opts o{};
+    o.device = device;
+    hp3d_pm_ctx* raw = nullptr;
+    check(hp3d_pm_create(&o, &raw), nullptr, "hp3d_pm_create");
+    std::unique_ptr<hp3d_pm_ctx, CtxDeleter> ctx(raw);
+    size_t n = 0;
+    check(hp3d_pm_fuse(ctx.get(), fv.data(), (int)fv.size(), min_confidence, nullptr, nullptr,
+                       nullptr, 0, &n),
+          ctx.get(), "fusion count");
+    pc.positions.resize(n);
+    pc.normals.resize(n);
+    pc.colors.resize(n);
+    if (n)
+        check(hp3d_pm_fuse(ctx.get(), fv.data(), (int)fv.size(), min_confidence,
+                           pc.positions[0].data(), pc.normals[0].data(), pc.colors[0].data(), n,
+                           &n),
+              ctx.get(), "fusion");
+    return pc;
+}
+
 bool saveDepthMapPFM(const DepthMap& dm, const std::string& path)
 {
     if (dm.width < 1 || dm.height < 1 || dm.depth.size() != (size_t)dm.width * dm.height)

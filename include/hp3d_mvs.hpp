@@ -10,6 +10,10 @@
 //   patchMatchMVS(...)               patchmatch_mvs.h:47-51 (legacy stub, .cu:1394-1404)
 //   runDenseReconstruction(...)      patchmatch_mvs.h:54-56 (legacy stub, .cu:1406-1410)
 //   saveDepthMapPFM / loadDepthMapPFM  patchmatch_mvs.h:59,62
+//   fuseDepthMaps(maps, poses, images) src/dense_recon/point_cloud.h:7-9 (a
+//                                    stub returning an empty cloud in the
+//                                    reference; here every valid pixel becomes
+//                                    a world point with normal and colour)
 //
 // Every device step goes through hp3d_pm.h; this layer holds no HIP code.
 // Differences in error behaviour: a device or argument failure throws
@@ -100,6 +104,12 @@ struct MVSViewParams {
     float depth_min = 0.0f, depth_max = 0.0f;
 };
 
+// DensePointCloud (include/types.h:93-105): host arrays only.
+struct DensePointCloud {
+    std::vector<std::array<float, 3>> positions, normals, colors;
+    int size() const { return (int)positions.size(); }
+};
+
 class MVSError : public std::runtime_error {
 public:
     MVSError(int code, const std::string& msg) : std::runtime_error(msg), code_(code) {}
@@ -122,6 +132,14 @@ DepthMap patchMatchMVS(const ImageData& ref_image, const std::vector<ImageData>&
 std::vector<DepthMap> runDenseReconstruction(const std::vector<ImageData>& images,
                                              const std::vector<CameraPose>& poses,
                                              const PipelineConfig& config);
+
+// Pixels with depth > 0 and confidence >= min_confidence (when the map has
+// a confidence channel) in map order then row-major order; colours from the
+// image's 8-bit gray (or gray * 255) of the map's image_id.
+DensePointCloud fuseDepthMaps(const std::vector<DepthMap>& depth_maps,
+                              const std::vector<CameraPose>& poses,
+                              const std::vector<ImageData>& images, float min_confidence = 0.0f,
+                              int device = 0);
 
 bool saveDepthMapPFM(const DepthMap& dm, const std::string& path);
 bool loadDepthMapPFM(DepthMap& dm, const std::string& path);

@@ -181,6 +181,31 @@ int hp3d_pm_geom_filter(hp3d_pm_ctx* ctx, hp3d_geom_view* views, int n, float de
                         float normal_tol_deg, int min_consistent, int* retained);
 
 /* ---------------------------------------------------------------------------
+ * Depth-map fusion into a point cloud.  The reference declares fuseDepthMaps
+ * (src/dense_recon/point_cloud.h:7-9) as the consumer of the MVS maps but
+ * ships a stub that returns an empty cloud (point_cloud.cu:4-8); this is its
+ * device form.  Every pixel with depth > 0 (and, where a confidence map is
+ * given, confidence >= min_confidence) becomes a point, in view order then
+ * row-major pixel order: position R^T (d K^-1 [x y 1] - t), normal R^T n,
+ * colour = 8-bit value / 255 (gray replicated to three channels, or RGB).
+ * Host memory in and out.  With positions == NULL only *count is reported;
+ * otherwise the arrays hold capacity points (3 floats each); normals and
+ * colors are nullable.
+ * ------------------------------------------------------------------------- */
+typedef struct {
+    int width, height;
+    const float* depth;         /* H*W, 0 = invalid */
+    const float* normal;        /* H*W*3 camera frame, nullable (zero normals) */
+    const float* confidence;    /* H*W, nullable */
+    const uint8_t* color;       /* H*W*color_channels, nullable */
+    int color_channels;         /* 1 (gray) or 3 (RGB) when color is given */
+    double K[9], R[9], t[3];
+} hp3d_fuse_view;
+
+int hp3d_pm_fuse(hp3d_pm_ctx* ctx, const hp3d_fuse_view* views, int n, float min_confidence,
+                 float* positions, float* normals, float* colors, size_t capacity, size_t* count);
+
+/* ---------------------------------------------------------------------------
  * Batched form: many reference views of any size, source count and
  * iteration count solved together.  The reference solves views one by one
  * in a host loop (patchmatch_mvs.cu:1076-1258) with a device synchronisation
@@ -197,7 +222,7 @@ typedef struct {
 int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
                       const hp3d_view_out* outs, const hp3d_batch_opts* opts);
 
-/* Autotuner: times every step-kernel variant (tile shape x waves per SIMD)
+/* Autotuner: times every step-kernel variant (tile shape and workgroup size)
  * on the given views for one start step and two half-passes, on scratch
  * state, and records the fastest for their (radius, stride, storage).
  * times_ms (nullable) receives HP3D_PM_NUM_VARIANTS entries, -1 where a

@@ -64,6 +64,12 @@ class OrcSeed(ctypes.Structure):
                 ("normal", ctypes.c_float * 3)]
 
 
+class OrcFuseView(ctypes.Structure):
+    _fields_ = [("W", ctypes.c_int), ("H", ctypes.c_int), ("depth", _f32p), ("normal", _f32p),
+                ("confidence", _f32p), ("color", _u8p), ("channels", ctypes.c_int),
+                ("kinv", ctypes.c_float * 9), ("R", ctypes.c_float * 9), ("t", ctypes.c_float * 3)]
+
+
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
 
@@ -120,6 +126,9 @@ def lib():
         L.orc_project_seeds.restype = ctypes.c_int
         L.orc_apply_seeds.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.POINTER(OrcSeed),
                                       ctypes.c_int, ctypes.c_int, _f32p, _f32p]
+        L.orc_fuse.argtypes = [ctypes.POINTER(OrcFuseView), ctypes.c_int, ctypes.c_float,
+                               _f32p, _f32p, _f32p]
+        L.orc_fuse.restype = ctypes.c_longlong
         _lib = L
     return _lib
 
@@ -342,3 +351,42 @@ def apply_seeds(depth, normal, seeds, radius=50):
             arr[i].normal[j] = float(n[j])
     assert depth.dtype == np.float32 and normal.dtype == np.float32
     lib().orc_apply_seeds(W, H, arr, len(seeds), int(radius), _p(depth, _f32p), _p(normal, _f32p))
+
+
+def fuse(maps, min_confidence=0.0):
+    """Restatement of the fusion; maps as for the product's Context.fuse.
+    K^-1 is inverted in FP64 and rounded to FP32, as are R and t."""
+    n = len(maps)
+    arr = (OrcFuseView * n)()
+    keep = []
+    total = 0
+    for i, m in enumerate(maps):
+        g = arr[i]
+        d = np.ascontiguousarray(m["depth"], np.float32)
+        keep.append(d)
+        g.H, g.W = d.shape
+        total += d.size
+        g.depth = _p(d, _f32p)
+        for k in ("normal", "confidence"):
+            if m.get(k) is not None:
+                a = np.ascontiguousarray(m[k], np.float32)
+                keep.append(a)
+                setattr(g, k, _p(a, _f32p))
+        if m.get("color") is not None:
+            c = np.ascontiguousarray(m["color"], np.uint8)
+            keep.append(c)
+            g.color = c.ctypes.data_as(_u8p)
+            g.channels = 1 if c.ndim == 2 else c.shape[2]
+        ki = inv3(np.asarray(m["K"], np.float64))
+        for j, val in enumerate(np.asarray(ki, np.float64).reshape(9)):
+            g.kinv[j] = float(np.float32(val))
+        for j, val in enumerate(np.asarray(m["R"], np.float64).reshape(9)):
+            g.R[j] = float(np.float32(val))
+        for j, val in enumerate(np.asarray(m["t"], np.float64).reshape(3)):
+            g.t[j] = float(np.float32(val))
+    pos = np.zeros((total, 3), np.float32)
+    nrm = np.zeros((total, 3), np.float32)
+    col = np.zeros((total, 3), np.float32)
+    N = lib().orc_fuse(arr, n, float(min_confidence), _p(pos, _f32p), _p(nrm, _f32p),
+                       _p(col, _f32p))
+    return pos[:N], nrm[:N], col[:N]

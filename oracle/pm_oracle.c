@@ -899,3 +899,61 @@ void orc_apply_seeds(int W, int H, const orc_seed* seeds, int n, int radius, flo
                 for (int i = 0; i < 3; ++i) normal[3 * q + i] = seeds[k].normal[i];
             }
 }
+
+/* ==========================================================================
+ * Depth-map fusion: pixel -> world point, camera normal -> world normal,
+ * 8-bit colour -> [0, 1], in view order then row-major order.
+ * ======================================================================== */
+long long orc_fuse(const orc_fuse_view* views, int n, float min_conf, float* pos, float* nrm,
+                   float* col)
+{
+    long long o = 0;
+    for (int i = 0; i < n; ++i) {
+        const orc_fuse_view* g = &views[i];
+        const float* ki = g->kinv;
+        const float* R = g->R;
+        for (int y = 0; y < g->H; ++y)
+            for (int x = 0; x < g->W; ++x) {
+                const size_t q = (size_t)y * g->W + x;
+                const float d = g->depth[q];
+                if (!(d > 0.0f)) continue;
+                if (g->confidence && !(g->confidence[q] >= min_conf)) continue;
+                if (pos) {
+                    const float fx = (float)x, fy = (float)y;
+                    const float pcx = d * (ki[0] * fx + ki[1] * fy + ki[2]);
+                    const float pcy = d * (ki[3] * fx + ki[4] * fy + ki[5]);
+                    const float pcz = d * (ki[6] * fx + ki[7] * fy + ki[8]);
+                    const float a = pcx - g->t[0], b = pcy - g->t[1], c = pcz - g->t[2];
+                    pos[3 * o] = R[0] * a + R[3] * b + R[6] * c;
+                    pos[3 * o + 1] = R[1] * a + R[4] * b + R[7] * c;
+                    pos[3 * o + 2] = R[2] * a + R[5] * b + R[8] * c;
+                }
+                if (nrm) {
+                    float nx = 0.0f, ny = 0.0f, nz = 0.0f;
+                    if (g->normal) {
+                        nx = g->normal[3 * q];
+                        ny = g->normal[3 * q + 1];
+                        nz = g->normal[3 * q + 2];
+                    }
+                    nrm[3 * o] = R[0] * nx + R[3] * ny + R[6] * nz;
+                    nrm[3 * o + 1] = R[1] * nx + R[4] * ny + R[7] * nz;
+                    nrm[3 * o + 2] = R[2] * nx + R[5] * ny + R[8] * nz;
+                }
+                if (col) {
+                    float r = 0.0f, gg = 0.0f, bb = 0.0f;
+                    if (g->color && g->channels == 1) {
+                        r = gg = bb = (float)g->color[q] / 255.0f;
+                    } else if (g->color && g->channels == 3) {
+                        r = (float)g->color[3 * q] / 255.0f;
+                        gg = (float)g->color[3 * q + 1] / 255.0f;
+                        bb = (float)g->color[3 * q + 2] / 255.0f;
+                    }
+                    col[3 * o] = r;
+                    col[3 * o + 1] = gg;
+                    col[3 * o + 2] = bb;
+                }
+                ++o;
+            }
+    }
+    return o;
+}

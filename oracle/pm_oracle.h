@@ -178,6 +178,22 @@ int orc_project_seeds(const orc_camera* cam, const orc_point* pts, int num_pts, 
 void orc_apply_seeds(int W, int H, const orc_seed* seeds, int n, int radius, float* depth,
                      float* normal);
 
+/* ---- depth-map fusion (the device form of fuseDepthMaps, which the
+ * reference declares at src/dense_recon/point_cloud.h:7-9 and stubs) ------- */
+typedef struct {
+    int W, H;
+    const float* depth;
+    const float* normal;      /* nullable */
+    const float* confidence;  /* nullable */
+    const uint8_t* color;     /* nullable */
+    int channels;
+    float kinv[9], R[9], t[3];
+} orc_fuse_view;
+/* Writes every valid pixel's point in view, row-major order; returns the
+ * count.  Arrays hold 3 floats per point; normals/colors nullable. */
+long long orc_fuse(const orc_fuse_view* views, int n, float min_conf, float* pos, float* nrm,
+                   float* col);
+
 #ifdef __cplusplus
 }
 #endif

@@ -396,6 +396,19 @@ static void EndToEndFilterAndAccuracy()
     EXPECT(rs >= 0.50, "sphere retention %f", rs);
     EXPECT(ra < 0.80, "total retention %f", ra);
     EXPECT(sec < 300.0, "%f s", sec);
+
+    // fusion of the filtered maps: every point near the sphere surface
+    DensePointCloud pc = fuseDepthMaps(dms, sfm_.cameras, images_);
+    EXPECT(pc.size() == (int)ret, "%d points for %lld retained pixels", pc.size(), ret);
+    long long near = 0;
+    for (const auto& p : pc.positions) {
+        const double r = std::sqrt(p[0] * p[0] + p[1] * p[1] + (p[2] - kCentre[2]) * (p[2] - kCentre[2]));
+        near += std::abs(r - kRadius) < 0.1 * kRadius;
+    }
+    const double fn = (double)near / std::max(1, pc.size());
+    std::printf("    [Test] Fusion: %d points, %.1f%% within 10%% of the sphere radius\n", pc.size(),
+                100.0 * fn);
+    EXPECT(fn > 0.5, "fraction near the surface %f", fn);
 }
 
 static void LegacyStubs()
