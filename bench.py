@@ -45,7 +45,7 @@ def parse():
     ap.add_argument("--scale", type=float, default=1.0, help="shrink W, H (not for reporting)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run one device-to-device copy of MIB MiB (counter calibration)")
-    ap.add_argument("--form", choices=["plain", "precision", "batched", "serving"],
+    ap.add_argument("--form", choices=["plain", "precision", "batched", "serving", "geom"],
                     default="plain",
                     help="plain: the headline line; the others print their own JSON line")
     ap.add_argument("--views", type=int, default=48, help="batched: views per batch")
@@ -92,7 +92,7 @@ def main():
     scenes = import_module(entry.PKG_NAME + ".scenes")
     if a.form != "plain":
         {"precision": form_precision, "batched": form_batched,
-         "serving": form_serving}[a.form](a, entry, pkg, scenes)
+         "serving": form_serving, "geom": form_geom}[a.form](a, entry, pkg, scenes)
         return
 
     job = scenes.c2(threads=a.cpu_threads, scale=a.scale)
@@ -124,6 +124,13 @@ def main():
     dt = time.perf_counter() - t0
     mpix_iters = a.steps * W * H * job.iterations / 1e6
     value = mpix_iters / dt
+    # the same solves with depth, normal and cost copied to host memory
+    # (PCIe-inclusive rate of the boundary; not the reported value)
+    n_host = max(1, min(a.steps, 5))
+    t0 = time.perf_counter()
+    for _ in range(n_host):
+        ctx.run_view(desc, job.shape)
+    host_value = n_host * W * H * job.iterations / 1e6 / (time.perf_counter() - t0)
 
     # roofline of the half-pass kernel
     taps_per_launch = (W * H / 2.0) * 6 * job.S * k
@@ -165,6 +172,7 @@ def main():
                        "p99": round(float(np.percentile(lat, 99)), 3)},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "host_outputs_value": round(host_value, 2),
     }
     print(json.dumps(out))
     ctx.close()
@@ -460,6 +468,55 @@ def form_serving(a, entry, pkg, scenes):
            "speedup_vs_plain_serial": round(value / (plain_mpix / plain_s), 3),
            "steps": s["steps"]}
     print(json.dumps(out))
+
+
+def form_geom(a, entry, pkg, scenes):
+    """The HBM-bound rows beside the hot path on the 36-view turntable at
+    1280x960 (ground-truth depth with 1 % noise, 8 sources per view):
+    geometric consistency filter over all views, then fusion into points.
+    Host buffers in and out, as the C ABI takes them (PCIe included); the
+    kernels' own durations come from rocprofv3."""
+    tt = scenes.turntable(threads=a.cpu_threads)
+    sfm = pkg.SfM(tt["cameras"], tt["points"])
+    vps = sfm.select_source_views(8)
+    rng = np.random.default_rng(3)
+    base = []
+    for v in range(len(tt["cameras"])):
+        d = tt["gt_depth"][v] * rng.normal(1.0, 0.01, tt["gt_depth"][v].shape).astype(np.float32)
+        d[tt["gt_depth"][v] <= 0] = rng.uniform(0.1, 0.4, int((tt["gt_depth"][v] <= 0).sum()))
+        base.append(d.astype(np.float32))
+    normal = np.zeros(base[0].shape + (3,), np.float32)
+    normal[..., 2] = -1.0
+    ctx = pkg.Context()
+
+    def maps():
+        return [dict(depth=base[v].copy(), normal=normal, confidence=np.ones_like(base[v]),
+                     K=c["K"], R=c["R"], t=c["t"], src=[s[0] for s in vps[v]["sources"]],
+                     color=tt["images"][v])
+                for v, c in enumerate(tt["cameras"])]
+
+    m = maps()
+    ctx.geom_filter(m)
+    reps, tg, tf = 3, 0.0, 0.0
+    for _ in range(reps):
+        m = maps()
+        t0 = time.perf_counter()
+        kept = ctx.geom_filter(m)
+        tg += time.perf_counter() - t0
+        t0 = time.perf_counter()
+        pos, _, _ = ctx.fuse(m)
+        tf += time.perf_counter() - t0
+    H, W = base[0].shape
+    mpix = len(base) * W * H / 1e6
+    out = {"form": "geom", "metric": "Mpixel/s (host buffers, PCIe included)",
+           "geom_filter_mpix_per_s": round(mpix / (tg / reps), 1),
+           "fuse_mpix_per_s": round(mpix / (tf / reps), 1),
+           "retained_fraction": round(float(sum(max(0, k) for k in kept)) / (len(base) * W * H), 4),
+           "points": int(len(pos)),
+           "config": {"workload": "36-view turntable 1280x960, 8 sources per view, GT depth "
+                                  "+1 % noise, random background depth"}}
+    print(json.dumps(out))
+    ctx.close()
 
 if __name__ == "__main__":
     main()

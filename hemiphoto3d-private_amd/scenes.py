@@ -258,8 +258,10 @@ def turntable_cameras(n_views=36, step_deg=10.0, W=1280, H=960, f=2000.0):
             (arc_camera(step_deg * i, center_z=0.25, radius=0.25) for i in range(n_views))]
 
 
-def turntable_points(cams, n_points=3000, W=1280, H=960, min_cos=0.2):
-    """Fibonacci-lattice points on the sphere with their observing views."""
+def turntable_points(cams, n_points=3000, W=1280, H=960, min_cos=0.5):
+    """Fibonacci-lattice points on the sphere with their observing views: a
+    view observes a point facing it within 60 deg (features much closer to
+    grazing are not matched by SIFT) that projects into its frame."""
     c = np.array([0.0, 0.0, 0.25])
     r = 0.05
     i = np.arange(n_points) + 0.5

@@ -119,7 +119,8 @@ static void setup_fixture()
     }
     sfm_.num_registered = kViews;
     // Fibonacci lattice on the sphere; a view observes a point that faces it
-    // (cos > 0.2) and projects into its frame
+    // within 60 degrees (cos > 0.5: features much closer to grazing are not
+    // matched by SIFT) and projects into its frame
     const int n = 3000;
     for (int k = 0; k < n; ++k) {
         const double i = k + 0.5, phi = std::acos(1.0 - 2.0 * i / n);
@@ -136,7 +137,7 @@ static void setup_fixture()
                 len += tc[a] * tc[a];
             }
             len = std::sqrt(len);
-            if ((nrm[0] * tc[0] + nrm[1] * tc[1] + nrm[2] * tc[2]) / len <= 0.2) continue;
+            if ((nrm[0] * tc[0] + nrm[1] * tc[1] + nrm[2] * tc[2]) / len <= 0.5) continue;
             double pc[3];
             for (int a = 0; a < 3; ++a)
                 pc[a] = c.R[3 * a] * p.position[0] + c.R[3 * a + 1] * p.position[1] +
