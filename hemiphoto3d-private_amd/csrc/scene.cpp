@@ -154,7 +154,9 @@ extern "C" int hp3d_scene_render(const hp3d_scene* scene, const hp3d_camera* cam
                 if (nc[0] * pc[0] + nc[1] * pc[1] + nc[2] * pc[2] > 0.0)
                     for (double& c : nc) c = -c;
             }
-            if (noise_sigma > 0.0) {   // Box-Muller on two hashed uniforms
+            // the sphere-only scene keeps its background flat (no texture to match)
+            if (noise_sigma > 0.0 && !(s.kind == HP3D_SCENE_SPHERE && th <= 0.0)) {
+                // Box-Muller on two hashed uniforms
                 const uint64_t h1 = mix64(noise_seed ^ mix64(2 * q)), h2 = mix64(noise_seed ^ mix64(2 * q + 1));
                 const double u1 = ((h1 >> 11) + 1) * (1.0 / 9007199254740992.0);
                 const double u2 = (h2 >> 11) * (1.0 / 9007199254740992.0);

@@ -24,9 +24,9 @@ extern "C" {
 typedef enum {
     HP3D_SCENE_PLANE = 0,         /* one infinite textured plane */
     HP3D_SCENE_SPHERE_GROUND = 1, /* sphere + ground plane y = ground_y + backdrop z = backdrop_z */
-    HP3D_SCENE_SPHERE = 2         /* sphere alone on a flat DN 30 background (depth 0 there),
-                                     the layout of the reference's turntable data
-                                     (src/utils/synthetic_data.cpp:17-28,80-84) */
+    HP3D_SCENE_SPHERE = 2         /* sphere alone on a flat DN 30 background (depth 0 and
+                                     no added noise there), the layout of the reference's
+                                     turntable data (src/utils/synthetic_data.cpp:17-28,80-84) */
 } hp3d_scene_kind;
 
 typedef struct {
