@@ -65,15 +65,8 @@ __global__ __launch_bounds__(TX * TY) void pm_cost_kernel(DevView v, int parity,
     cost_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, threadIdx.x % TX, threadIdx.x / TX, parity);
 }
 
-// Build-time tuning knobs of the per-view kernels (tile shape, minimum waves
-// per SIMD the register allocation must allow); the defaults are the fastest
-// measured on the 1600x1200 workload.
-#ifndef HP3D_PLAIN_TX
-#define HP3D_PLAIN_TX 32
-#endif
-#ifndef HP3D_PLAIN_TY
-#define HP3D_PLAIN_TY 8
-#endif
+// Build-time tuning knob of the per-view kernels: minimum waves per SIMD the
+// register allocation must allow (1: no constraint).
 #ifndef HP3D_PLAIN_WPE
 #define HP3D_PLAIN_WPE 1
 #endif
@@ -188,13 +181,24 @@ hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_
     return hipGetLastError();
 }
 
-// 32 compacted columns (64 pixels) x 8 rows per workgroup by default.
-constexpr int kTX = HP3D_PLAIN_TX, kTY = HP3D_PLAIN_TY;
+// Workgroup tile (compacted columns x rows): 32 x 8 (64 x 8 pixels) for the
+// 6-tap rows of a stride-2 window, 64 x 4 for the 11-tap rows of stride 1 --
+// the fastest of the tile sweeps on 1600x1200 views (HP3D_PLAIN_TX / _TY
+// override both at build time).
+template <int NT>
+struct PlainTile {
+#ifdef HP3D_PLAIN_TX
+    static constexpr int TX = HP3D_PLAIN_TX, TY = HP3D_PLAIN_TY;
+#else
+    static constexpr int TX = NT >= 9 ? 64 : 32, TY = NT >= 9 ? 4 : 8;
+#endif
+};
 
 template <int NT, int ST, int STORE>
 static void launch_pass_t(const DevView& v, int mode, int iteration, int parity,
                           uint64_t pass_seed, hipStream_t st)
 {
+    constexpr int kTX = PlainTile<NT>::TX, kTY = PlainTile<NT>::TY;
     const int tiles_x = (v.Wc + kTX - 1) / kTX;
     const int tiles = tiles_x * ((v.H + kTY - 1) / kTY);
     const dim3 grid(((tiles + 7) / 8) * 8);

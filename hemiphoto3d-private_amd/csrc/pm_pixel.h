@@ -186,7 +186,11 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
     constexpr int R = (NT - 1) * ST / 2;
     float sr = 0.f, ss = 0.f, srr = 0.f, sss = 0.f, srs = 0.f;
     int cnt = 0;
-#pragma unroll 1
+    // window rows: unrolled by two for the 11-tap (stride 1) rows, where the
+    // extra loads in flight pay; one at a time for the 6-tap rows, where
+    // unrolling measured slower
+    constexpr int kRowUnroll = NT >= 9 ? 2 : 1;
+#pragma unroll kRowUnroll
     for (int j = 0; j < NT; ++j) {
         const float oy = (float)(j * ST - R);
         const float bx = fmaf(h01, oy, cx);
