@@ -41,11 +41,11 @@ struct DevSrc {
 
 struct DevView {
     const void* ref_plain;              // reference plane
-    const void* src_plain[kMaxSrc];     // source planes (F32 storage)
-    const void* src_coef[kMaxSrc];      // source coefficient planes (F16 storage)
+    const void* src_plain[kMaxSrc];     // source planes
+    const void* src_coef[kMaxSrc];      // source gather planes: F32 row pairs, F16 coefficients
     int pitch_plain;                    // elements per row of a plane (both kinds)
     int plain_bytes;                    // bytes of one plane (buffer range check)
-    int coef_bytes;                     // bytes of one coefficient plane
+    int coef_bytes;                     // bytes of one gather plane
     float kinv[9];                      // K_ref^-1 (row-major), hypothesis arithmetic
     double kinv_d[9];                   // K_ref^-1 for the cost
     DevSrc src[kMaxSrc];

@@ -109,9 +109,10 @@ __global__ __launch_bounds__(256) void pm_import_kernel(DevView v, const float* 
         make_float4(depth[q], normal[3 * q], normal[3 * q + 1], normal[3 * q + 2]);
 }
 
-// 8-bit or [0,1] float gray -> the image planes (DN - 128, see pm_device.h).
-// in_u8 / in_f32: exactly one is non-null; in_pitch in bytes.  coef is used
-// by F16 storage only.
+// 8-bit or [0,1] float gray -> the image planes (DN - 128, see pm_device.h):
+// the plain plane and the gather plane (F32: row pairs (I[y][x], I[y+1][x]),
+// F16: bilinear coefficients).  in_u8 / in_f32: exactly one is non-null;
+// in_pitch in bytes.
 template <int STORE>
 __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, const float* in_f32,
                                                        size_t in_pitch, int W, int H, void* plain,
@@ -129,6 +130,7 @@ __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, con
     const size_t q = (size_t)y * pitch + x;
     if (STORE == kStoreF32) {
         static_cast<float*>(plain)[q] = a;
+        static_cast<float2*>(coef)[q] = make_float2(a, y < H - 1 ? val(x, y + 1) : 0.0f);
     } else {
         static_cast<_Float16*>(plain)[q] = (_Float16)a;
         if (x < W - 1 && y < H - 1) {

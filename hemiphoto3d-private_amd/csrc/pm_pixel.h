@@ -95,30 +95,28 @@ __device__ __forceinline__ float lerp_fma(float t, float a, float b)
 }
 
 // Bilinear sample at (ix + au, iy + aw) of the texel quad a = I[iy][ix],
-// b = I[iy][ix+1], c = I[iy+1][ix], d = I[iy+1][ix+1].
-//   F32: the plain plane; one 8-byte load per texel row fetches the pair
-//        (ix, ix+1), the second row uses the same offset plus a scalar row
-//        pitch; two row lerps and a column lerp.  4 bytes per texel.
+// b = I[iy][ix+1], c = I[iy+1][ix], d = I[iy+1][ix+1].  Both storages keep a
+// second plane for source gathers, so that one 16- or 8-byte load per tap
+// fetches the whole quad:
+//   F32: row-pair plane, element (x, y) = (I[y][x], I[y+1][x]) as two floats;
+//        the load at (ix, iy) returns (a, c, b, d); two row lerps and a
+//        column lerp, exactly as on the plain plane.  8 bytes per pixel.
 //   F16: the coefficient plane (a, B, C, D) = (a, b-a, c-a, (a-b)-(c-d)) as
-//        four halves; one 8-byte load per tap and a + au*B + aw*(C + au*D),
-//        the half->float conversions folded into v_fma_mix_f32.
-// (Measured on the 1600x1200 4-source workload, these layouts beat the
-// other combinations: the number of loads per tap matters as much as bytes.)
+//        four halves; a + au*B + aw*(C + au*D), the half->float conversions
+//        folded into v_fma_mix_f32.  8 bytes per pixel.
+// (Measured on the 1600x1200 4-source workload: the number of loads per tap
+// matters as much as the bytes.)
 template <int STORE> struct SrcImage;
 template <> struct SrcImage<kStoreF32> {
     __amdgpu_buffer_rsrc_t rsrc;
     unsigned pitch;
-    int row_bytes;
     __device__ SrcImage(const DevView& v, int s)
-        : rsrc(make_rsrc(v.src_plain[s], v.plain_bytes)), pitch((unsigned)v.pitch_plain),
-          row_bytes(v.pitch_plain * 4) {}
+        : rsrc(make_rsrc(v.src_coef[s], v.coef_bytes)), pitch((unsigned)v.pitch_plain) {}
     __device__ __forceinline__ float sample(int ix, int iy, float au, float aw) const
     {
-        const unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 2;
-        const float2 t = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
-        const float2 u =
-            __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, row_bytes, 0));
-        return lerp_fma(aw, lerp_fma(au, t.x, t.y), lerp_fma(au, u.x, u.y));
+        const unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 3;
+        const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+        return lerp_fma(aw, lerp_fma(au, q.x, q.z), lerp_fma(au, q.y, q.w));
     }
 };
 template <> struct SrcImage<kStoreF16> {
