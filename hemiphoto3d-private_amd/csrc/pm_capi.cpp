@@ -2,7 +2,7 @@
 //
 // Replaces the per-view driver of runPatchMatchMVS
 // (src/dense_recon/patchmatch_mvs.cu:1028-1258): images live in HBM as
-// plain + bilinear-coefficient planes, hypothesis state as two colour
+// plain + row-pair planes, hypothesis state as two colour
 // planes, and every launch of a view goes onto the context's stream with no
 // host synchronisation until the outputs are needed (the reference
 // synchronises after every launch, src/utils/cuda_utils.cuh:18-22).
@@ -305,7 +305,7 @@ int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int W, int H, size_
         return ctx->fail(HP3D_ERR_UNSUPPORTED, "image %dx%d: plane above 2 GiB", W, H);
     int rc = ctx->dev_alloc(&img.plain, img.plain_bytes, "image plane");
     if (rc) return rc;
-    rc = ctx->dev_alloc(&img.coef, img.coef_bytes, "image coefficient plane");
+    rc = ctx->dev_alloc(&img.coef, img.coef_bytes, "image row-pair plane");
     if (rc) { ctx->dev_free(img.plain, img.plain_bytes); return rc; }
     auto release = [&] {
         ctx->dev_free(img.plain, img.plain_bytes);

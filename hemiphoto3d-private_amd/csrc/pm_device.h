@@ -4,14 +4,12 @@
 // Data layout in HBM (one reference view being solved):
 //   * images: plain[y][x] = DN - 128, row pitch a multiple of 64 elements.
 //     The shift keeps window sums small (the NCC is invariant to it).
-//     F32 storage: the float plane only (4 B/px); a bilinear tap reads the
-//     texel pairs (x, x+1) of rows y and y+1 with one 8-byte load each.
-//     F16 storage: a _Float16 plane (reference windows) plus a coefficient
-//     plane coef[y][x] = (a, b-a, c-a, (a-b)-(c-d)) of the quad
-//     a = plain[y][x], b = plain[y][x+1], c = plain[y+1][x],
-//     d = plain[y+1][x+1] as four halves (10 B/px in all); a tap is one
-//     8-byte load.  For 8-bit input every value is an integer in
-//     [-510, 510], so F16 storage is lossless.
+//     Beside the plain plane (reference windows, staged in LDS) every image
+//     has a row-pair plane pair[y][x] = (plain[y][x], plain[y+1][x]) for
+//     source gathers: one load at (x, y) returns the whole bilinear quad.
+//     F32 storage: 4 + 8 B/px; F16 storage: 2 + 4 B/px.  For 8-bit input
+//     every value is an integer in [-128, 127], exact in FP16, so F16
+//     storage is lossless and gives the same results as F32.
 //   * hypotheses: two compacted colour planes (red: (x+y) even, black: odd),
 //     each H x Wc with Wc = (W+1)/2.  Pixel (x, y) of colour c sits at
 //     column i = x >> 1 of plane c.  hyp = (depth, nx, ny, nz) as float4,
@@ -42,7 +40,7 @@ struct DevSrc {
 struct DevView {
     const void* ref_plain;              // reference plane
     const void* src_plain[kMaxSrc];     // source planes
-    const void* src_coef[kMaxSrc];      // source gather planes: F32 row pairs, F16 coefficients
+    const void* src_coef[kMaxSrc];      // source gather planes (row pairs)
     int pitch_plain;                    // elements per row of a plane (both kinds)
     int plain_bytes;                    // bytes of one plane (buffer range check)
     int coef_bytes;                     // bytes of one gather plane

@@ -110,9 +110,8 @@ __global__ __launch_bounds__(256) void pm_import_kernel(DevView v, const float* 
 }
 
 // 8-bit or [0,1] float gray -> the image planes (DN - 128, see pm_device.h):
-// the plain plane and the gather plane (F32: row pairs (I[y][x], I[y+1][x]),
-// F16: bilinear coefficients).  in_u8 / in_f32: exactly one is non-null;
-// in_pitch in bytes.
+// the plain plane and the row-pair gather plane (I[y][x], I[y+1][x]), in
+// FP32 or FP16.  in_u8 / in_f32: exactly one is non-null; in_pitch in bytes.
 template <int STORE>
 __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, const float* in_f32,
                                                        size_t in_pitch, int W, int H, void* plain,
@@ -133,15 +132,10 @@ __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, con
         static_cast<float2*>(coef)[q] = make_float2(a, y < H - 1 ? val(x, y + 1) : 0.0f);
     } else {
         static_cast<_Float16*>(plain)[q] = (_Float16)a;
-        if (x < W - 1 && y < H - 1) {
-            const float b = val(x + 1, y), c = val(x, y + 1), d = val(x + 1, y + 1);
-            half4_t h;
-            h.x = (_Float16)a;
-            h.y = (_Float16)(b - a);
-            h.z = (_Float16)(c - a);
-            h.w = (_Float16)((a - b) - (c - d));
-            static_cast<half4_t*>(coef)[q] = h;
-        }
+        half2_t h;
+        h.x = (_Float16)a;
+        h.y = (_Float16)(y < H - 1 ? val(x, y + 1) : 0.0f);
+        static_cast<half2_t*>(coef)[q] = h;
     }
 }
 
@@ -183,16 +177,15 @@ hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_
     return hipGetLastError();
 }
 
-// Workgroup tile (compacted columns x rows): 32 x 8 (64 x 8 pixels) for the
-// 6-tap rows of a stride-2 window, 64 x 4 for the 11-tap rows of stride 1 --
-// the fastest of the tile sweeps on 1600x1200 views (HP3D_PLAIN_TX / _TY
-// override both at build time).
+// Workgroup tile (compacted columns x rows): 32 x 8 (64 x 8 pixels), the
+// fastest of the 16x16 / 32x8 / 64x4 sweeps on 1600x1200 views at both
+// strides (HP3D_PLAIN_TX / _TY override it at build time).
 template <int NT>
 struct PlainTile {
 #ifdef HP3D_PLAIN_TX
     static constexpr int TX = HP3D_PLAIN_TX, TY = HP3D_PLAIN_TY;
 #else
-    static constexpr int TX = NT >= 9 ? 64 : 32, TY = NT >= 9 ? 4 : 8;
+    static constexpr int TX = 32, TY = 8;
 #endif
 };
 

@@ -13,8 +13,8 @@
 
 namespace hp3d {
 
-// One uploaded image: its plane of DN - 128 values and the gather plane of
-// source taps (F32: row pairs, F16: bilinear coefficients; pm_pixel.h).
+// One uploaded image: its plane of DN - 128 values and the row-pair gather
+// plane of source taps (pm_pixel.h).
 struct Image {
     int W = 0, H = 0, store = 0;
     int pitch_plain = 0;                   // elements, both planes
@@ -30,7 +30,7 @@ inline void image_plane_sizes(int W, int H, int store, int* pitch, size_t* plain
 {
     *pitch = (W + 63) & ~63;
     *plain_bytes = (size_t)*pitch * H * (store == HP3D_STORE_F32 ? 4 : 2);
-    *coef_bytes = (size_t)*pitch * H * 8;   // 8 bytes per pixel for both storages
+    *coef_bytes = (size_t)*pitch * H * (store == HP3D_STORE_F32 ? 8 : 4);   // row pairs
 }
 
 struct DevBuf {
