@@ -79,11 +79,12 @@ void pm_halfpass_kernel(DevView v, int iteration, int parity, uint64_t pass_seed
     constexpr int R = (NT - 1) * ST / 2;
     using G = TileGeom<R, STORE, TX, TY>;
     __shared__ float tile[G::LH * G::LP];
+    __shared__ WaveXchg xw[TX * TY / 64];
     int bx, by;
     if (!xcd_tile(tiles_x, tiles, bx, by)) return;
     stage_ref_tile<R, STORE, TX, TY>(v, tile, bx, by);
     halfpass_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, threadIdx.x % TX, threadIdx.x / TX,
-                                          iteration, parity, pass_seed);
+                                          iteration, parity, pass_seed, xw[threadIdx.x >> 6]);
 }
 
 // Colour planes -> row-major depth (H*W), AoS normal (H*W*3), cost and

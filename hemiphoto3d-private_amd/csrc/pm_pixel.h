@@ -403,50 +403,129 @@ __device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, 
     return true;
 }
 
+// Per-wave LDS scratch of a half-pass: the wave's list of neighbour
+// candidates that must be evaluated (owner lane | candidate << 6), and their
+// costs.  At most 4 per lane.
+struct WaveXchg {
+    unsigned char item[256];
+    float cost[256];
+};
+
 // One pixel of a red (parity 0) or black (parity 1) half-pass:
 // propagate_and_refine_kernel (:519-655).  Candidates L, R, U, D from the
 // other colour plane, then two random refinements with ranges
 // (dmax-dmin)/2 * 0.6^iter and 0.6^iter, halved per step.  Strict '<'
 // acceptance throughout.
+//
+// A neighbour candidate that is bit-identical to the pixel's own hypothesis,
+// or to an earlier valid candidate, cannot be accepted: its cost is the same
+// deterministic function of (pixel, hypothesis), and bc is already at most
+// that cost.  Once propagation has spread planes, 40-60 % of the neighbour
+// candidates are such repeats, but rarely all 64 lanes of a wave at once.  So
+// the wave lists the candidates its lanes still need, any lane evaluates any
+// pixel's candidate (all windows of the workgroup are in LDS), and each
+// owner then applies the costs in L, R, U, D order.  The result is the one
+// the plain per-lane loop gives, with fewer evaluation rounds per wave.
+// Every thread of the workgroup must call this (it synchronises the group).
 template <int NT, int ST, int STORE, int TX, int TY>
 __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* tile, int bx, int by,
                                                int tx, int ty, int iteration, int parity,
-                                               uint64_t pass_seed)
+                                               uint64_t pass_seed, WaveXchg& xw)
 {
     constexpr int R = (NT - 1) * ST / 2;
     using G = TileGeom<R, STORE, TX, TY>;
+    const int lane = threadIdx.x & 63;
+    const int wave0 = threadIdx.x & ~63;
     const int i = bx * TX + tx;
     const int y = by * TY + ty;
     const int o = (y + parity) & 1;
     const int x = 2 * i + o;
-    if (y >= v.H || x >= v.W) return;
+    const bool active = y < v.H && x < v.W;
     const float* lwin = tile + ty * G::LP + (x - 2 * bx * TX);
-    double rx, ry, rz;
-    ray_fp64(v.kinv_d, x, y, rx, ry, rz);
     const float dmin = v.dmin, dmax = v.dmax;
     const int Wc = v.Wc;
-
     float4* own = v.hyp[parity];
     const float4* oth = v.hyp[parity ^ 1];
     const int p = y * Wc + i;
-    float4 best = own[p];
-    float bc = v.cost[parity][p];
-    const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
-
     // neighbour positions in the other plane: L = i+o-1, R = i+o (row y),
     // U, D = i (rows y-1, y+1)
-#pragma unroll 1
+    auto nb = [&](int k, int pp, int oo) {
+        return k == 0 ? pp + oo - 1 : k == 1 ? pp + oo : k == 2 ? pp - Wc : pp + Wc;
+    };
+    auto same = [](const float4& a, const float4& b) {
+        return __float_as_uint(a.x) == __float_as_uint(b.x) &&
+               __float_as_uint(a.y) == __float_as_uint(b.y) &&
+               __float_as_uint(a.z) == __float_as_uint(b.z) &&
+               __float_as_uint(a.w) == __float_as_uint(b.w);
+    };
+
+    float4 best = make_float4(0.f, 0.f, 0.f, 0.f);
+    float bc = 2.0f;
+    unsigned need = 0;   // bit k: neighbour k has to be evaluated
+    if (active) {
+        best = own[p];
+        bc = v.cost[parity][p];
+        float4 h[4];
+        bool ok[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const bool has = k == 0 ? x >= 1 : k == 1 ? x + 1 < v.W : k == 2 ? y >= 1 : y + 1 < v.H;
+            ok[k] = false;
+            h[k] = best;
+            if (has) {
+                h[k] = oth[nb(k, p, o)];
+                ok[k] = !(h[k].x < dmin || h[k].x > dmax);
+            }
+            bool n = ok[k] && !same(h[k], best);
+#pragma unroll
+            for (int j = 0; j < k; ++j) n = n && !(ok[j] && same(h[j], h[k]));
+            need |= (unsigned)n << k;
+        }
+    }
+    const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
+
+    // wave-wide list ordered by candidate, then lane
+    const unsigned long long below = (1ull << lane) - 1ull;
+    int slot[4];
+    unsigned total = 0;
+#pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const bool has = k == 0 ? x >= 1 : k == 1 ? x + 1 < v.W : k == 2 ? y >= 1 : y + 1 < v.H;
-        if (!has) continue;
-        const int q = k == 0 ? p + o - 1 : k == 1 ? p + o : k == 2 ? p - Wc : p + Wc;
-        const float4 h = oth[q];
-        if (h.x < dmin || h.x > dmax) continue;
-        const float c = cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x, y, rx, ry, rz, h.x, h.y,
-                                                 h.z, h.w);
-        if (c < bc) { bc = c; best = h; }
+        const bool n = (need >> k) & 1u;
+        const unsigned long long m = __ballot(n);
+        slot[k] = (int)(total + __popcll(m & below));
+        total += (unsigned)__popcll(m);
+        if (n) xw.item[slot[k]] = (unsigned char)(lane | (k << 6));
+    }
+    __syncthreads();
+    for (unsigned r = 0; r < total; r += 64) {
+        const unsigned g = r + lane;
+        const unsigned it = xw.item[g < total ? g : total - 1];
+        const int ol = it & 63, k = it >> 6;
+        const WinRef wro{__shfl(wr.sr, ol, 64), __shfl(wr.srr, ol, 64)};
+        if (g < total) {
+            const int t_o = wave0 + ol;
+            const int tx_o = t_o % TX, ty_o = t_o / TX;
+            const int i_o = bx * TX + tx_o, y_o = by * TY + ty_o;
+            const int o_o = (y_o + parity) & 1, x_o = 2 * i_o + o_o;
+            const float4 h = oth[nb(k, y_o * Wc + i_o, o_o)];
+            double rx, ry, rz;
+            ray_fp64(v.kinv_d, x_o, y_o, rx, ry, rz);
+            xw.cost[g] = cand_cost<NT, ST, STORE>(v, tile + ty_o * G::LP + (x_o - 2 * bx * TX),
+                                                  G::LP, wro, x_o, y_o, rx, ry, rz, h.x, h.y, h.z,
+                                                  h.w);
+        }
+    }
+    __syncthreads();
+    if (!active) return;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        if (!((need >> k) & 1u)) continue;
+        const float c = xw.cost[slot[k]];
+        if (c < bc) { bc = c; best = oth[nb(k, p, o)]; }
     }
 
+    double rx, ry, rz;
+    ray_fp64(v.kinv_d, x, y, rx, ry, rz);
     float ux, uy, uz;
     unit_ray(v.kinv, x, y, ux, uy, uz);
     float depth_range = (dmax - dmin) * 0.5f;

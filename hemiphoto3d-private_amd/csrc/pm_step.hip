@@ -36,6 +36,7 @@ void pm_step_kernel(const DevJob* __restrict__ jobs, const StepItem* __restrict_
     constexpr int R = (NT - 1) * ST / 2;
     using G = TileGeom<R, STORE, TX, TY>;
     __shared__ float tile[G::LH * G::LP];
+    __shared__ WaveXchg xw[TX * TY / 64];
 
     const int b = blockIdx.x;                       // gridDim.x: a multiple of 8 * kStepChunk
     const int sidx = b >> 3;                        // position in this XCD's sequence
@@ -67,7 +68,7 @@ void pm_step_kernel(const DevJob* __restrict__ jobs, const StepItem* __restrict_
         cost_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, tx, ty, 1);
     } else {
         halfpass_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, tx, ty, it.iteration, it.parity,
-                                              it.pass_seed);
+                                              it.pass_seed, xw[threadIdx.x >> 6]);
     }
     // outputs: this lane's pixel of each colour; the other colour was
     // completed by the previous step
