@@ -264,7 +264,8 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
     for (int k = 0; k < kMaxTopK; ++k) top[k] = 2.0f;
 
     const unsigned umax = (unsigned)(v.W - 2), vmax = (unsigned)(v.H - 2);
-    const float Wm1 = (float)(v.W - 1), Hm1 = (float)(v.H - 1);
+    constexpr float kEdgeMargin = 4e-3f;
+    const float Wm = (float)(v.W - 1) - kEdgeMargin, Hm = (float)(v.H - 1) - kEdgeMargin;
 
     for (int s = 0; s < v.S; ++s) {
         const double* A = v.src[s].A;
@@ -279,23 +280,23 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
         const SrcImage<STORE> img(v, s);
 
         // Every tap of this lane is inside the image if the four window
-        // corners are: w_z is affine in the tap offset, so one sign at the
+        // corners are: w_z is affine in the tap offset, so w_z > 0 at the
         // corners means no pole inside the window, and a projective map
         // without a pole keeps the window inside its corners' convex hull.
-        // The 4e-3 px margin covers FP32 rounding of interior taps.
+        // With w_z > 0, u = w_x / w_z in [m, W-1-m] is w_x in
+        // [m w_z, (W-1-m) w_z]: no division.  The m = 4e-3 px margin covers
+        // FP32 rounding of interior taps.  (Windows with w_z < 0 everywhere
+        // take the masked loop, which gives the same result.)
         bool inside = true;
-        float sz0 = 0.0f;
 #pragma unroll
         for (int q = 0; q < 4; ++q) {
             const float oy = (q & 2) ? (float)R : (float)-R, ox = (q & 1) ? (float)R : (float)-R;
             const float wx = fmaf(h00, ox, fmaf(h01, oy, cx));
             const float wy = fmaf(h10, ox, fmaf(h11, oy, cy));
             const float wz = fmaf(h20, ox, fmaf(h21, oy, cz));
-            const float rw = __builtin_amdgcn_rcpf(wz);
-            const float u = wx * rw, w = wy * rw;
-            if (q == 0) sz0 = wz;
-            inside = inside & (fabsf(wz) >= 1e-7f) & ((wz > 0.0f) == (sz0 > 0.0f)) &
-                     (u >= 4e-3f) & (u <= Wm1 - 4e-3f) & (w >= 4e-3f) & (w <= Hm1 - 4e-3f);
+            const float lo = kEdgeMargin * wz;
+            inside = inside & (wz >= 1e-7f) & (wx >= lo) & (wx <= Wm * wz) & (wy >= lo) &
+                     (wy <= Hm * wz);
         }
         TapSums m;
         if (__all(inside))
