@@ -292,6 +292,8 @@ int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int W, int H, size_
     const size_t row = (size_t)W * (fmt == HP3D_FMT_U8 ? 1 : 4);
     if (pitch == 0) pitch = row;
     if (pitch < row) return ctx->fail(HP3D_ERR_INVALID_ARG, "pitch %zu < row bytes %zu", pitch, row);
+    if (fmt == HP3D_FMT_F32 && pitch % 4 != 0)
+        return ctx->fail(HP3D_ERR_INVALID_ARG, "float image pitch %zu is not a multiple of 4", pitch);
     (void)hipSetDevice(ctx->device);
 
     Image img;
@@ -312,17 +314,18 @@ int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int W, int H, size_
         ctx->dev_free(img.coef, img.coef_bytes);
     };
 
-    // stage the host rows on the device, then convert in one kernel
-    const size_t stage_bytes = row * H;
+    // stage the host rows, with their pitch, on the device in one copy (a
+    // 2-D copy from pageable memory goes row by row), then convert in one
+    // kernel that reads the pitched rows
+    const size_t stage_bytes = pitch * (size_t)(H - 1) + row;
     rc = ctx->grow(ctx->staging, stage_bytes, "upload staging");
     if (rc) { release(); return rc; }
     void* stage = ctx->staging.ptr;
-    hipError_t e = hipMemcpy2DAsync(stage, row, data, pitch, row, H, hipMemcpyHostToDevice,
-                                    ctx->stream);
+    hipError_t e = hipMemcpyAsync(stage, data, stage_bytes, hipMemcpyHostToDevice, ctx->stream);
     if (e == hipSuccess)
         e = launch_img_prep(fmt == HP3D_FMT_U8 ? (const uint8_t*)stage : nullptr,
-                            fmt == HP3D_FMT_F32 ? (const float*)stage : nullptr, row, W, H, store,
-                            img.plain, img.coef, img.pitch_plain, ctx->stream);
+                            fmt == HP3D_FMT_F32 ? (const float*)stage : nullptr, pitch, W, H,
+                            store, img.plain, img.coef, img.pitch_plain, ctx->stream);
     if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     if (e != hipSuccess) {
         release();

@@ -98,3 +98,43 @@ def test_source_that_sees_nothing(pkg, orc, ctx, scenes):
     assert np.abs(c.astype(np.float64) - c64)[ok].max() <= COST_TOL
     for h in handles:
         ctx.release(h)
+
+
+def test_pitched_uploads_equal_packed(pkg, ctx, scenes):
+    """Images handed over with padded rows (any byte pitch for 8-bit, a
+    multiple of 4 for float) give the same planes as packed rows."""
+    import ctypes
+    job = scenes.sphere_job(75, 41, 2, radius=2, stride=1, iterations=1, name="pitch")
+    H, W = job.shape
+
+    def upload_pitched(img, pad, store):
+        buf = np.zeros((H, W + pad), dtype=img.dtype)
+        buf[:, :W] = img
+        buf[:, W:] = 255 if img.dtype == np.uint8 else 7.0      # garbage in the padding
+        out = ctypes.c_uint64(0)
+        fmt = pkg.FMT_U8 if img.dtype == np.uint8 else pkg.FMT_F32
+        rc = pkg.lib().hp3d_pm_image_upload(ctx.h, buf.ctypes.data, W, H, buf.strides[0], fmt,
+                                            store, ctypes.byref(out))
+        assert rc == 0, ctx.last_error()
+        return out.value
+
+    for store in (0, 1):
+        packed = [ctx.upload(job.ref, store)] + [ctx.upload(s, store) for s in job.srcs]
+        d = pkg.make_desc(packed[0], packed[1:], job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                          radius=2, stride=1, seed=job.seed)
+        d0, n0 = ctx.init_hypotheses(d, job.shape)
+        want = ctx.cost_eval(d, d0, n0)
+        for imgs, pad in (([job.ref] + job.srcs, 13),
+                          ([(i.astype(np.float32) / 255.0) for i in [job.ref] + job.srcs], 3)):
+            h = [upload_pitched(i, pad, store) for i in imgs]
+            dp = pkg.make_desc(h[0], h[1:], job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                               radius=2, stride=1, seed=job.seed)
+            got = ctx.cost_eval(dp, d0, n0)
+            if imgs[0].dtype == np.uint8:
+                assert np.array_equal(got, want)
+            else:   # gray / 255 * 255 is within an ulp of the DN
+                assert np.abs(got - want).max() <= 1e-5
+            for x in h:
+                ctx.release(x)
+        for x in packed:
+            ctx.release(x)
