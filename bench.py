@@ -12,7 +12,14 @@ roofline: the dominant kernel is the half-pass (pm_halfpass_kernel).
   algorithmic work per launch = (W*H/2 active pixels) * 6 candidates * S
   sources * k taps * 34 FLOP/tap; its average launch duration is measured
   with HIP events recorded by the library on its own stream during the timed
-  steps.  Peak: 157.3 TFLOP/s (MI355X FP32 vector, datasheet).
+  steps.  Peak: 157.3 TFLOP/s (MI355X FP32 vector, datasheet).  The
+  kernel issues no MFMA and is bound by VALU issue, so bound = "valu".
+  traffic, hbm_gbs, l2_read_gbs: HBM bytes (FETCH_SIZE + WRITE_SIZE) and
+  L2-to-L1 read bytes (TCP_TCC_READ_REQ) per launch from rocprofv3 counter
+  runs of this workload, calibrated on a 1 GiB copy and kept in
+  profiles/traffic_halfpass_c2.json, over the live launch time; peaks
+  8 TB/s (HBM3E) and 39.3 TB/s (L2: 8 XCDs x 16 channels x 128 B/clk x
+  2.4 GHz).
 cpu_baseline: the oracle's FP32 restatement of the reference algorithm on
   host threads, timed on a bounded sample of the same view (2 iterations),
   scaled through its measured tap rate to this workload's taps per
@@ -30,6 +37,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3
+PEAK_HBM_GBS = 8000.0
 FLOP_PER_TAP = 34
 TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_halfpass_c2.json")
 
@@ -136,14 +144,24 @@ def main():
     taps_per_launch = (W * H / 2.0) * 6 * job.S * k
     avg_launch_s = prop_ms / prop_n / 1e3
     achieved = taps_per_launch * FLOP_PER_TAP / avg_launch_s / 1e12
-    traffic = None
+    traffic, prof = None, {}
     if os.path.exists(TRAFFIC_FILE) and a.scale == 1.0 and a.store == "f32":
-        traffic = json.load(open(TRAFFIC_FILE)).get("hbm_bytes_per_launch")
+        prof = json.load(open(TRAFFIC_FILE))
+        traffic = prof.get("hbm_bytes_per_launch")
     roofline = {"bound": "valu", "achieved": round(achieved, 3), "peak": PEAK_FP32_TFLOPS,
                 "unit": "TFLOP/s", "frac": round(achieved / PEAK_FP32_TFLOPS, 4),
                 "traffic": traffic, "kernel": "pm_halfpass_kernel",
                 "avg_launch_ms": round(avg_launch_s * 1e3, 4),
                 "flop_per_launch": taps_per_launch * FLOP_PER_TAP}
+    # memory-side rates of the same launches: counter bytes per launch (profiled
+    # once, rocprofv3) over the live launch time
+    if traffic:
+        gbs = traffic / avg_launch_s / 1e9
+        roofline.update(hbm_gbs=round(gbs, 1), hbm_frac=round(gbs / PEAK_HBM_GBS, 4))
+    if prof.get("l2_read_bytes_per_launch"):
+        l2 = prof["l2_read_bytes_per_launch"] / avg_launch_s / 1e9
+        roofline.update(l2_read_gbs=round(l2, 1), l2_frac=round(l2 / prof["l2_peak_gbs"], 4),
+                        l2_hit_rate=prof.get("l2_hit_rate"), l1_hit_rate=prof.get("l1_hit_rate"))
 
     cpu = None
     if not a.no_cpu:

@@ -137,23 +137,27 @@ def test_f16_storage_of_float_images_within_tolerance(pkg, orc, ctx, sphere_smal
 # ---------------------------------------------------------------------------
 # full runs vs the oracle's schedule
 # ---------------------------------------------------------------------------
-def _run_compare(pkg, orc, ctx, job, iterations, label):
+def _run_compare(pkg, orc, ctx, job, iterations, label, with_fp32=True, threads=None):
     ref, srcs = upload_job(ctx, job)
     out = ctx.run_view(desc_for(pkg, job, ref, srcs, iterations=iterations), job.shape)
+    for h in [ref] + srcs:
+        ctx.release(h)
     v = oracle_view(orc, job)
-    d64, _, _ = v.patchmatch_fp32(iterations, job.seed, cost_mode=orc.COST_FP64)
-    d32, _, _ = v.patchmatch_fp32(iterations, job.seed, cost_mode=orc.COST_FP32)
+    d64, _, _ = v.patchmatch_fp32(iterations, job.seed, threads=threads, cost_mode=orc.COST_FP64)
+    runs = [("gpu", out["depth"]), ("fp64", d64)]
+    if with_fp32:
+        d32, _, _ = v.patchmatch_fp32(iterations, job.seed, threads=threads,
+                                      cost_mode=orc.COST_FP32)
+        runs.append(("fp32", d32))
     dg = out["depth"]
     gpu_vs64 = rel_within(dg, d64).mean()
-    gpu_vs32 = rel_within(dg, d32).mean()
-    o32_vs64 = rel_within(d32, d64).mean()
+    gpu_vs32 = rel_within(dg, d32).mean() if with_fp32 else None
+    o32_vs64 = rel_within(d32, d64).mean() if with_fp32 else None
     gt = job.gt_depth
     valid = gt > 0
-    acc = {k: rel_within(d[valid], gt[valid]).mean() for k, d in (("gpu", dg), ("fp64", d64),
-                                                                 ("fp32", d32))}
-    mre = {k: np.minimum(np.abs(d[valid] - gt[valid]) / gt[valid], 1.0).mean()
-           for k, d in (("gpu", dg), ("fp64", d64), ("fp32", d32))}
-    print(f"{label}: gpu~fp64 {gpu_vs64:.4f} gpu~fp32 {gpu_vs32:.4f} fp32~fp64 {o32_vs64:.4f} "
+    acc = {k: rel_within(d[valid], gt[valid]).mean() for k, d in runs}
+    mre = {k: np.minimum(np.abs(d[valid] - gt[valid]) / gt[valid], 1.0).mean() for k, d in runs}
+    print(f"{label}: gpu~fp64 {gpu_vs64:.4f} gpu~fp32 {gpu_vs32} fp32~fp64 {o32_vs64} "
           f"| GT<1%: {acc} | mean rel err: {mre}")
     return out, gpu_vs64, gpu_vs32, o32_vs64, acc, mre
 
@@ -174,6 +178,21 @@ def test_depth_parity_converged_c1(pkg, orc, ctx, c1):
     """C1 at 8 iterations: also >= 99 % against the reference-order FP32 run."""
     out, g64, g32, o3264, acc, mre = _run_compare(pkg, orc, ctx, c1, 8, "c1-8it")
     assert g64 >= 0.99 and g32 >= 0.99
+
+
+def test_depth_parity_c2_full_size(pkg, orc, ctx, scenes):
+    """The benchmark workload itself (1600x1200, 4 sources, 11x11 at stride 2,
+    8 iterations): >= 99 % of pixels within 1 % of the FP64-cost oracle run,
+    error against the ground truth no worse.  The oracle takes about half a
+    minute on 16 host threads here."""
+    import os
+    job = scenes.c2()
+    threads = min(16, os.cpu_count() or 1)
+    out, g64, _, _, acc, mre = _run_compare(pkg, orc, ctx, job, job.iterations, "c2-full",
+                                            with_fp32=False, threads=threads)
+    assert g64 >= 0.99
+    assert acc["gpu"] >= acc["fp64"] - 0.003
+    assert mre["gpu"] <= mre["fp64"] * 1.02 + 1e-4
 
 
 # ---------------------------------------------------------------------------
