@@ -209,11 +209,41 @@ def cpu_baseline(entry, job, threads, k):
     taps = W * H * (job.S * k + iters * 6 * job.S * k)
     taps_per_s = taps / dt
     taps_per_mpix_iter = (job.S * k + job.iterations * 6 * job.S * k) / job.iterations * 1e6
-    return {"value": round(taps_per_s / taps_per_mpix_iter, 3), "unit": "Mpixel-iters/s",
-            "cores": threads, "kind": "port",
-            "sample": f"same {W}x{H} view, {iters} iterations ({taps / 1e9:.2f} G taps) in "
-                      f"{dt:.1f} s; scaled by the tap rate to the {job.iterations}-iteration workload",
-            "taps_per_s": round(taps_per_s)}
+    out = {"value": round(taps_per_s / taps_per_mpix_iter, 3), "unit": "Mpixel-iters/s",
+           "cores": threads, "kind": "port",
+           "sample": f"same {W}x{H} view, {iters} iterations ({taps / 1e9:.2f} G taps) in "
+                     f"{dt:.1f} s; scaled by the tap rate to the {job.iterations}-iteration workload",
+           "taps_per_s": round(taps_per_s)}
+    out.update(host_cpu())
+    # the CPU-runnable workload (320x240, 2 sources, 11x11 stride 1, 3
+    # iterations) run whole, on one core and on `threads`
+    from importlib import import_module
+    c1 = import_module(entry.PKG_NAME + ".scenes").c1(threads=threads)
+    v1 = orc.View(c1.ref, c1.srcs, c1.Kinv, c1.A, c1.b, c1.dmin, c1.dmax, radius=c1.radius,
+                  stride=c1.stride, top_k=c1.top_k)
+    H1, W1 = c1.shape
+    for n in (1, threads):
+        t0 = time.perf_counter()
+        v1.patchmatch_fp32(c1.iterations, c1.seed, threads=n, cost_mode=orc.COST_FP32)
+        dt1 = time.perf_counter() - t0
+        out[f"c1_{n}core" + ("s" if n > 1 else "")] = {
+            "value": round(W1 * H1 * c1.iterations / dt1 / 1e6, 4), "unit": "Mpixel-iters/s",
+            "seconds": round(dt1, 2), "taps_per_s": round(c1.taps() / dt1)}
+    return out
+
+
+def host_cpu():
+    """CPU model and logical CPU count of the host the baseline ran on."""
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"cpu_model": model, "nproc": os.cpu_count()}
 
 
 

@@ -229,6 +229,41 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
 }
 
 // ---------------------------------------------------------------------------
+// True if the tap loop, with its FP32 arithmetic, will find every tap of the
+// window valid (|w_z| >= 1e-8, 0 <= u < W-1, 0 <= v < H-1), so the unmasked
+// loop gives the masked loop's result.  A tap's w = c + ox h_0 + oy h_1 is
+// affine in the offset, so over the square |ox|, |oy| <= R its extremes are
+// c -/+ R (|a| + |b|): no corners to enumerate.  The conditions used are
+//   min w_x > 0, min w_y > 0, min w_z > 1e-8      (then u, v >= 0), and
+//   max (w_x - Wq w_z) < 0, max (w_y - Hq w_z) < 0,
+// with Wq = (W-1)(1 - 2^-20), so that u = w_x * rcp(w_z) < W-1 even with
+// v_rcp_f32's 1-ulp error and the rounding of the product.  Each bound is
+// tightened by 2^-21 (8 FP32 rounding units) times the magnitude of the terms
+// it is built from (T = |c| + R (|a| + |b|)): a tap's coordinate carries at
+// most 2 units of that (two FMAs), the bound's own evaluation at most 4.  So
+// the test is exact in what it promises, also for grazing windows where the
+// terms nearly cancel.  NaN or infinite inputs fail every comparison.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ bool window_inside(float cx, float cy, float cz, float h00, float h01,
+                                              float h10, float h11, float h20, float h21, float R,
+                                              float Wq, float Hq)
+{
+    constexpr float kE = 4.76837158203125e-07f;   // 2^-21
+    const float ax = fabsf(h00) + fabsf(h01), ay = fabsf(h10) + fabsf(h11);
+    const float az = fabsf(h20) + fabsf(h21);
+    const float Tx = fmaf(R, ax, fabsf(cx)), Ty = fmaf(R, ay, fabsf(cy));
+    const float Tz = fmaf(R, az, fabsf(cz));
+    const float lox = fmaf(-R, ax, cx), loy = fmaf(-R, ay, cy), loz = fmaf(-R, az, cz);
+    const float gx = fmaf(R, fabsf(fmaf(-Wq, h20, h00)) + fabsf(fmaf(-Wq, h21, h01)),
+                          fmaf(-Wq, cz, cx));
+    const float gy = fmaf(R, fabsf(fmaf(-Hq, h20, h10)) + fabsf(fmaf(-Hq, h21, h11)),
+                          fmaf(-Hq, cz, cy));
+    return (fmaf(-kE, Tx, lox) >= 0.0f) && (fmaf(-kE, Ty, loy) >= 0.0f) &&
+           (fmaf(-kE, Tz, loz) >= 1e-8f) && (fmaf(kE, fmaf(Wq, Tz, Tx), gx) < 0.0f) &&
+           (fmaf(kE, fmaf(Hq, Tz, Ty), gy) < 0.0f);
+}
+
+// ---------------------------------------------------------------------------
 // Cost of one candidate plane (d, n) at pixel (x, y): patchmatch_mvs.cu:222-343.
 // lwin points at the pixel's window top-left tap in the LDS reference tile.
 // (rx, ry, rz) is the unnormalised ray K^-1 [x, y, 1] in FP64.
@@ -264,8 +299,9 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
     for (int k = 0; k < kMaxTopK; ++k) top[k] = 2.0f;
 
     const unsigned umax = (unsigned)(v.W - 2), vmax = (unsigned)(v.H - 2);
-    constexpr float kEdgeMargin = 4e-3f;
-    const float Wm = (float)(v.W - 1) - kEdgeMargin, Hm = (float)(v.H - 1) - kEdgeMargin;
+    constexpr float kShrink = 0.99999904632568359375f;   // 1 - 2^-20
+    const float Wq = (float)(v.W - 1) * kShrink, Hq = (float)(v.H - 1) * kShrink;
+    const bool small_k = v.top_k <= 3;   // wave-uniform
 
     for (int s = 0; s < v.S; ++s) {
         const double* A = v.src[s].A;
@@ -279,25 +315,11 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
         const float h20 = (float)fma(b[2], g0, A[6]), h21 = (float)fma(b[2], g1, A[7]);
         const SrcImage<STORE> img(v, s);
 
-        // Every tap of this lane is inside the image if the four window
-        // corners are: w_z is affine in the tap offset, so w_z > 0 at the
-        // corners means no pole inside the window, and a projective map
-        // without a pole keeps the window inside its corners' convex hull.
-        // With w_z > 0, u = w_x / w_z in [m, W-1-m] is w_x in
-        // [m w_z, (W-1-m) w_z]: no division.  The m = 4e-3 px margin covers
-        // FP32 rounding of interior taps.  (Windows with w_z < 0 everywhere
-        // take the masked loop, which gives the same result.)
-        bool inside = true;
-#pragma unroll
-        for (int q = 0; q < 4; ++q) {
-            const float oy = (q & 2) ? (float)R : (float)-R, ox = (q & 1) ? (float)R : (float)-R;
-            const float wx = fmaf(h00, ox, fmaf(h01, oy, cx));
-            const float wy = fmaf(h10, ox, fmaf(h11, oy, cy));
-            const float wz = fmaf(h20, ox, fmaf(h21, oy, cz));
-            const float lo = kEdgeMargin * wz;
-            inside = inside & (wz >= 1e-7f) & (wx >= lo) & (wx <= Wm * wz) & (wy >= lo) &
-                     (wy <= Hm * wz);
-        }
+        // The unmasked tap loop when every lane's window is inside the
+        // source image (windows with w_z < 0 throughout take the masked
+        // loop, which gives the same result).
+        const bool inside = window_inside(cx, cy, cz, h00, h01, h10, h11, h20, h21, (float)R,
+                                          Wq, Hq);
         TapSums m;
         if (__all(inside))
             tap_window<NT, ST, STORE, false>(img, lwin, lp, cx, cy, cz, h00, h01, h10, h11, h20,
@@ -314,15 +336,24 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
             const double dr = fc * (double)srr - (double)sr * (double)sr;
             const double ds = fc * (double)sss - (double)ss * (double)ss;
             if (!(dr < (double)v.den_eps) && !(ds < (double)v.den_eps)) {
-                float ncc = (float)num * rsqrtf((float)dr * (float)ds);
+                // dr, ds >= den_eps (~0.065): far from the denormal range,
+                // where rsqrtf would first rescale, so the bare v_rsq_f32
+                float ncc = (float)num * __builtin_amdgcn_rsqf((float)dr * (float)ds);
                 ncc = fmaxf(-1.0f, fminf(1.0f, ncc));
                 c = 1.0f - ncc;
             }
         }
-        // insert into the ascending list of the kMaxTopK smallest costs
+        // insert into the ascending list of the smallest costs: 3 entries
+        // suffice when at most 3 are averaged (the reference's top-3), else 8
+        if (small_k) {
 #pragma unroll
-        for (int k = kMaxTopK - 1; k > 0; --k)
-            top[k] = (c < top[k - 1]) ? top[k - 1] : (c < top[k] ? c : top[k]);
+            for (int k = 2; k > 0; --k)
+                top[k] = (c < top[k - 1]) ? top[k - 1] : (c < top[k] ? c : top[k]);
+        } else {
+#pragma unroll
+            for (int k = kMaxTopK - 1; k > 0; --k)
+                top[k] = (c < top[k - 1]) ? top[k - 1] : (c < top[k] ? c : top[k]);
+        }
         top[0] = c < top[0] ? c : top[0];
     }
     const int k = v.top_k < v.S ? v.top_k : v.S;
