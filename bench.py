@@ -327,6 +327,15 @@ def form_precision(a, entry, pkg, scenes):
            "roofline": {"bound": "valu", "achieved": res["fp16"]["halfpass_tflops"],
                         "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                         "frac": round(res["fp16"]["halfpass_tflops"] / PEAK_FP32_TFLOPS, 4)}}
+    if not a.no_cpu:
+        # too large to run on the host: extrapolated from the measured tap rate
+        rate = _cpu_tap_rate(entry, a.cpu_threads)
+        cpu = rate / job.taps() * job.mpix_iters()
+        out["cpu_baseline"] = {"value": round(cpu, 3), "unit": "Mpixel-iters/s",
+                               "cores": a.cpu_threads, "kind": "port",
+                               "sample": "640x480 4-source view, 1 iteration, extrapolated by "
+                                         "taps to this view"}
+        out["speedup_vs_cpu"] = round(out["value"] / cpu, 1)
     print(json.dumps(out))
     ctx.close()
 
