@@ -479,6 +479,9 @@ def form_serving(a, entry, pkg, scenes):
     for jid, p, it in pending:
         st = srv.wait(jid, timeout=600)
         failed += st is None or st["state"] != pkg.JOB_DONE
+        if st is None:                 # still running: its buffers stay with the server
+            srv.cancel(jid)
+            continue
         collect(st, p, it)
         srv.forget(jid)
     s = srv.stats()

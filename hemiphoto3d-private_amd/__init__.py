@@ -210,6 +210,7 @@ SIGNATURES = {
     "hp3d_server_wait": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_double,
                                         ctypes.POINTER(JobStatus)]),
     "hp3d_server_cancel": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "hp3d_server_forget": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
     "hp3d_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
     "hp3d_server_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
     "hp3d_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
@@ -765,6 +766,9 @@ class Server:
         return self._jobs[jid]
 
     def forget(self, jid):
+        """Drop a finished job (its record in the server and its output
+        arrays here); HP3DError(ERR_AGAIN) while it is queued or running."""
+        _check(lib().hp3d_server_forget(self.h, jid))
         self._jobs.pop(jid, None)
 
     def cancel(self, jid):

@@ -109,7 +109,7 @@ struct hp3d_pm_server {
     std::mutex mu;
     std::condition_variable cv_work, cv_done;
     std::deque<std::shared_ptr<SJob>> queue;
-    std::map<uint64_t, std::shared_ptr<SJob>> jobs;   // every job ever submitted
+    std::map<uint64_t, std::shared_ptr<SJob>> jobs;   // submitted and not yet forgotten
     std::vector<std::shared_ptr<SJob>> active;
     uint64_t next_id = 0;
     bool stop = false;
@@ -456,6 +456,18 @@ int hp3d_server_cancel(hp3d_pm_server* s, uint64_t id)
     if (it->second->st.state >= HP3D_JOB_DONE) return HP3D_OK;
     it->second->cancel = true;
     s->cv_work.notify_all();
+    return HP3D_OK;
+}
+
+int hp3d_server_forget(hp3d_pm_server* s, uint64_t id)
+{
+    if (!s) return HP3D_ERR_INVALID_ARG;
+    std::lock_guard<std::mutex> lk(s->mu);
+    auto it = s->jobs.find(id);
+    if (it == s->jobs.end()) return HP3D_ERR_NOT_FOUND;
+    // a queued or running job may still write the caller's output buffers
+    if (it->second->st.state < HP3D_JOB_DONE) return HP3D_ERR_AGAIN;
+    s->jobs.erase(it);
     return HP3D_OK;
 }
 

@@ -309,6 +309,12 @@ int hp3d_server_poll(hp3d_pm_server* srv, uint64_t id, hp3d_job_status* st);
  * HP3D_ERR_AGAIN on timeout. */
 int hp3d_server_wait(hp3d_pm_server* srv, uint64_t id, double timeout_s, hp3d_job_status* st);
 int hp3d_server_cancel(hp3d_pm_server* srv, uint64_t id);
+/* Drop the record of a finished (done, failed or cancelled) job: later polls
+ * return HP3D_ERR_NOT_FOUND.  HP3D_ERR_AGAIN while the job is queued or
+ * running -- its output buffers must stay valid until then.  A long-running
+ * server keeps one small record per job until it is forgotten.  (No
+ * counterpart in the reference, which has no serving loop.) */
+int hp3d_server_forget(hp3d_pm_server* srv, uint64_t id);
 int hp3d_server_get_stats(hp3d_pm_server* srv, hp3d_server_stats* st);
 const char* hp3d_server_last_error(hp3d_pm_server* srv);
 int hp3d_server_destroy(hp3d_pm_server* srv);

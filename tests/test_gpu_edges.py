@@ -138,3 +138,51 @@ def test_pitched_uploads_equal_packed(pkg, ctx, scenes):
                 ctx.release(x)
         for x in packed:
             ctx.release(x)
+
+
+@pytest.mark.parametrize("top_k", [1, 2, 4, 5, 8])
+def test_top_k_variants_match_oracle(pkg, orc, ctx, scenes, top_k):
+    """Every top-k the boundary accepts (1..8; the reference fixes 3), with 8
+    sources: the kernels keep a 3-entry list when at most 3 costs are
+    averaged and an 8-entry one otherwise.  The selection and averaging are
+    what this checks, on a scene chosen to be hard for FP32 moments (7x7
+    windows of low contrast far from mid-gray): there a single source's
+    cost differs from FP64 by up to 2.4e-4, all of it from accumulating the
+    moments in FP32 (a numpy restatement of the kernel's arithmetic with
+    FP64 moments agrees to 3e-6), and costs averaged over k sources carry
+    much of it.  So the tolerance here is 4e-4 -- against ~3e-3 for the
+    reference's own FP32 operation order, which the GPU must also beat 4x
+    on a sample; the 1e-4 gate of the workloads is tested in
+    test_gpu_parity.py, test_gpu_precision.py and above.  A 2-iteration run
+    must put >= 99 % of pixels within 1 % of the oracle's depth."""
+    tol = 4e-4
+    job = scenes.sphere_job(160, 120, 8, radius=3, stride=1, iterations=2, scene_seed=31,
+                            name="topk")
+    v = oracle_view(orc, job, top_k=top_k)
+    for store in (0, 1):
+        d, handles = _desc(pkg, ctx, job, store, top_k=top_k)
+        d0, n0 = ctx.init_hypotheses(d, job.shape)
+        c = ctx.cost_eval(d, d0, n0)
+        c64, flags = v.cost_map_fp64(d0, n0)
+        ok = flags == 0
+        assert ok.mean() > 0.6
+        err = np.abs(c.astype(np.float64) - c64)
+        ys, xs = np.nonzero(ok)
+        pick = np.random.default_rng(top_k).choice(len(xs), 1500, replace=False)
+        ref32 = max(abs(v.cost_fp32(int(xs[i]), int(ys[i]), float(d0[ys[i], xs[i]]),
+                                    n0[ys[i], xs[i]]) - c64[ys[i], xs[i]]) for i in pick)
+        wy, wx = np.unravel_index(np.argmax(np.where(ok, err, 0)), err.shape)
+        per_src = np.sort(np.asarray(v.cost_fp64(int(wx), int(wy), float(d0[wy, wx]), n0[wy, wx],
+                                                 per_src=True)[2]))
+        print(f"top_k={top_k} store={store}: max|err| {err[ok].max():.2e} at ({wx},{wy}): gpu "
+              f"{c[wy, wx]:.7f} fp64 {c64[wy, wx]:.7f}, per-source fp64 {np.round(per_src, 6)}; "
+              f"reference-order FP32 on a sample {ref32:.2e}")
+        assert err[ok].max() <= tol, (top_k, store)
+        assert err[ys[pick], xs[pick]].max() <= 0.25 * ref32
+        out = ctx.run_view(d, job.shape)
+        for h in handles:
+            ctx.release(h)
+    o64, _, _ = v.patchmatch_fp32(job.iterations, job.seed, cost_mode=orc.COST_FP64)
+    agree = (np.abs(out["depth"].astype(np.float64) - o64) <= 0.01 * np.abs(o64)).mean()
+    print(f"top_k={top_k}: depth within 1 % of the FP64 oracle on {agree:.4f}")
+    assert agree >= 0.99

@@ -110,6 +110,28 @@ def test_cancel_and_injected_failure_leave_others_intact(pkg, serve_jobs, expect
         assert (s["done"], s["failed"], s["cancelled"]) == (2, 1, 1)
 
 
+def test_forget_only_finished_jobs(pkg, serve_jobs, expected):
+    """A queued or running job cannot be forgotten (its output buffers are
+    still to be written); a finished one is dropped and no longer found."""
+    with pkg.Server(512 * MB, max_jobs=1, stride=2) as srv:
+        first = srv.submit_job(serve_jobs[2])
+        queued = srv.submit_job(serve_jobs[0])      # waits for the only slot
+        with pytest.raises(pkg.HP3DError) as e:
+            srv.forget(queued)
+        assert e.value.code == pkg.ERR_AGAIN
+        assert srv.wait(first, 120)["state"] == pkg.JOB_DONE
+        assert srv.wait(queued, 120)["state"] == pkg.JOB_DONE
+        _check(srv.result(queued), expected[0])
+        srv.forget(first)
+        srv.forget(queued)
+        with pytest.raises(pkg.HP3DError) as e:
+            srv.poll(first)
+        assert e.value.code == pkg.ERR_NOT_FOUND
+        with pytest.raises(pkg.HP3DError) as e:
+            srv.forget(first)
+        assert e.value.code == pkg.ERR_NOT_FOUND
+
+
 def test_invalid_submissions(pkg, serve_jobs):
     with pkg.Server(64 * MB, max_jobs=2, stride=2) as srv:
         j = serve_jobs[0]
