@@ -516,17 +516,31 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
     }
     const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
 
-    // wave-wide list ordered by candidate, then lane
+    // Wave-wide list ordered by owner lane, then candidate: a round of 64
+    // items then covers ~25 neighbouring pixels whose candidates project to
+    // nearly the same source rows and columns, so its gathers touch far
+    // fewer 64-B segments than 64 pixels' worth would (the L1 access rate,
+    // not bandwidth, bounds this kernel).
     const unsigned long long below = (1ull << lane) - 1ull;
     int slot[4];
     unsigned total = 0;
+    int base = 0;   // items of the lanes below this one
 #pragma unroll
     for (int k = 0; k < 4; ++k) {
-        const bool n = (need >> k) & 1u;
-        const unsigned long long m = __ballot(n);
-        slot[k] = (int)(total + __popcll(m & below));
+        const unsigned long long m = __ballot((need >> k) & 1u);
+        base += __popcll(m & below);
         total += (unsigned)__popcll(m);
-        if (n) xw.item[slot[k]] = (unsigned char)(lane | (k << 6));
+    }
+    {
+        int off = base;
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            slot[k] = off;
+            if ((need >> k) & 1u) {
+                xw.item[off] = (unsigned char)(lane | (k << 6));
+                ++off;
+            }
+        }
     }
     __syncthreads();
     for (unsigned r = 0; r < total; r += 64) {
