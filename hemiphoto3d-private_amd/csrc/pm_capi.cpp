@@ -303,8 +303,8 @@ int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int W, int H, size_
     img.den_scale = 255.0f * 255.0f;
     image_plane_sizes(W, H, store, &img.pitch_plain, &img.plain_bytes, &img.coef_bytes);
     // taps address a plane with 32-bit buffer offsets
-    if (img.plain_bytes > (size_t)0x7fffffff || img.coef_bytes > (size_t)0x7fffffff)
-        return ctx->fail(HP3D_ERR_UNSUPPORTED, "image %dx%d: plane above 2 GiB", W, H);
+    if (img.plain_bytes > kMaxPlaneBytes || img.coef_bytes > kMaxPlaneBytes)
+        return ctx->fail(HP3D_ERR_UNSUPPORTED, "image %dx%d: plane above 2 GiB - 64 KiB", W, H);
     int rc = ctx->dev_alloc(&img.plain, img.plain_bytes, "image plane");
     if (rc) return rc;
     rc = ctx->dev_alloc(&img.coef, img.coef_bytes, "image row-pair plane");

@@ -26,6 +26,9 @@ constexpr int kMaxTopK = 8;
 constexpr int kMaxSeedRadius = 4096;
 constexpr int kMaxSeedCoord = 1 << 24;
 constexpr float kShiftDN = 128.0f;
+// Largest image plane: taps address a plane with 32-bit buffer offsets, and
+// offsets at or above this bound are a guaranteed miss of every plane.
+constexpr unsigned long long kMaxPlaneBytes = 0x7fff0000ull;
 
 enum Store : int { kStoreF32 = 0, kStoreF16 = 1 };
 

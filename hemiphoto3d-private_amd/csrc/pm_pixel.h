@@ -82,6 +82,10 @@ template <> __device__ __forceinline__ float load_plain<kStoreF16>(const void* p
 // never fault.  Dword 3 = 0x00020000 is the raw 32-bit data format of
 // gfx9-family buffer descriptors.
 constexpr int kBufferRsrcWord3 = 0x00020000;
+// An offset past every plane (planes hold at most kMaxPlaneBytes): the load
+// is range-checked away and returns zeros.
+constexpr unsigned kOffNowhere = 0x7ffffff0u;
+static_assert(kOffNowhere >= kMaxPlaneBytes, "offset inside a plane");
 
 __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, int bytes)
 {
@@ -110,9 +114,10 @@ template <> struct SrcImage<kStoreF32> {
     unsigned pitch;
     __device__ SrcImage(const DevView& v, int s)
         : rsrc(make_rsrc(v.src_coef[s], v.coef_bytes)), pitch((unsigned)v.pitch_plain) {}
-    __device__ __forceinline__ float sample(int ix, int iy, float au, float aw) const
+    __device__ __forceinline__ float sample(int ix, int iy, float au, float aw, bool ok = true) const
     {
-        const unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 3;
+        unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 3;
+        off = ok ? off : kOffNowhere;
         const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
         return lerp_fma(aw, lerp_fma(au, q.x, q.z), lerp_fma(au, q.y, q.w));
     }
@@ -122,9 +127,10 @@ template <> struct SrcImage<kStoreF16> {
     unsigned pitch;
     __device__ SrcImage(const DevView& v, int s)
         : rsrc(make_rsrc(v.src_coef[s], v.coef_bytes)), pitch((unsigned)v.pitch_plain) {}
-    __device__ __forceinline__ float sample(int ix, int iy, float au, float aw) const
+    __device__ __forceinline__ float sample(int ix, int iy, float au, float aw, bool ok = true) const
     {
-        const unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 2;
+        unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 2;
+        off = ok ? off : kOffNowhere;
         const half4_t q = __builtin_bit_cast(half4_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
         return lerp_fma(aw, lerp_fma(au, (float)q.x, (float)q.z), lerp_fma(au, (float)q.y, (float)q.w));
     }
@@ -203,13 +209,13 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
             const float u = wx * rw, w = wy * rw;
             const int ix = cvt_flr(u), iy = cvt_flr(w);
             const float au = __builtin_amdgcn_fractf(u), aw = __builtin_amdgcn_fractf(w);
-            float sval = img.sample(ix, iy, au, aw);
+            // |w_z| guard and 0 <= u < W-1, 0 <= v < H-1 (:289-295); an
+            // invalid tap's load goes nowhere and reads zeros (so sval = 0)
+            const bool ok = !MASKED || ((fabsf(wz) >= 1e-8f) & ((unsigned)ix <= umax) &
+                                        ((unsigned)iy <= vmax));
+            const float sval = img.sample(ix, iy, au, aw, ok);
             float r = lrow[i * ST];
             if (MASKED) {
-                // |w_z| guard and 0 <= u < W-1, 0 <= v < H-1 (:289-295)
-                const bool ok = (fabsf(wz) >= 1e-8f) & ((unsigned)ix <= umax) &
-                                ((unsigned)iy <= vmax);
-                sval = ok ? sval : 0.0f;
                 r = ok ? r : 0.0f;
                 cnt += ok ? 1 : 0;
             }

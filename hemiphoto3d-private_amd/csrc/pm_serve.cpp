@@ -401,8 +401,8 @@ int hp3d_server_submit(hp3d_pm_server* s, const hp3d_job_desc* d, uint64_t* id)
     int pitch;
     size_t plain_bytes, coef_bytes;
     image_plane_sizes(d->width, d->height, s->o.store, &pitch, &plain_bytes, &coef_bytes);
-    if (j->footprint > s->arena.size() || plain_bytes > 0x7fffffffull ||
-        coef_bytes > 0x7fffffffull) {
+    if (j->footprint > s->arena.size() || plain_bytes > kMaxPlaneBytes ||
+        coef_bytes > kMaxPlaneBytes) {
         s->last_error = "submit: job footprint exceeds the server's HBM budget";
         s->stats.rejected++;
         return HP3D_ERR_OOM_BUDGET;
