@@ -125,7 +125,9 @@ const char* hp3d_pm_last_error(const hp3d_pm_ctx* ctx);
 /* Upload one gray image.  fmt U8: 8-bit DN; fmt F32: gray in [0,1] as in
  * ImageData::gray (include/types.h:17).  pitch in bytes (0: packed rows; a
  * multiple of 4 for F32); the pitch * (height - 1) + row bytes from data are
- * copied.  Replaces createGrayscaleTexture (patchmatch_mvs.cu:789-814). */
+ * copied.  HP3D_ERR_UNSUPPORTED if a device plane of the image would exceed
+ * 2 GiB - 64 KiB (about 16000 x 16000 px with FP32 storage).  Replaces
+ * createGrayscaleTexture (patchmatch_mvs.cu:789-814). */
 int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int width, int height,
                          size_t pitch, int fmt, int store, hp3d_img* out);
 int hp3d_pm_image_release(hp3d_pm_ctx* ctx, hp3d_img img);
