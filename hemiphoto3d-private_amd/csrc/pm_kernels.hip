@@ -178,15 +178,18 @@ hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_
     return hipGetLastError();
 }
 
-// Workgroup tile (compacted columns x rows): 32 x 8 (64 x 8 pixels), the
-// fastest of the 16x16 / 32x8 / 64x4 sweeps on 1600x1200 views at both
-// strides (HP3D_PLAIN_TX / _TY override it at build time).
+// Workgroup tile (compacted columns x rows), from sweeps on 1600x1200 views:
+// 32 x 8 (64 x 8 pixels) for windows of up to 8 taps a side (6 x 6 at
+// stride 2: 32x8, 32x4 and 32x2 within 1 %, 16x16 and 64x4 slower); 32 x 4
+// for the 11 x 11 stride-1 window, whose kernel holds 3 waves per SIMD:
+// half-size workgroups leave a shorter tail per launch, +8.5 % with FP32
+// storage (HP3D_PLAIN_TX / _TY override both at build time).
 template <int NT>
 struct PlainTile {
 #ifdef HP3D_PLAIN_TX
     static constexpr int TX = HP3D_PLAIN_TX, TY = HP3D_PLAIN_TY;
 #else
-    static constexpr int TX = 32, TY = 8;
+    static constexpr int TX = 32, TY = NT >= 9 ? 4 : 8;
 #endif
 };
 
