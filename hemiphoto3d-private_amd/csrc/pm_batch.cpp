@@ -24,7 +24,8 @@ static size_t align256(size_t n) { return (n + 255) & ~(size_t)255; }
 size_t job_state_bytes(int W, int H, int want_outputs_mask)
 {
     const size_t plane = (size_t)H * ((W + 1) / 2);
-    size_t n = align256(2 * plane * sizeof(float4)) + align256(2 * plane * sizeof(float));
+    size_t n = align256(2 * plane * sizeof(float4)) + align256(2 * plane * sizeof(float)) +
+               align256(2 * plane);
     const size_t px = (size_t)W * H * sizeof(float);
     if (want_outputs_mask & 1) n += align256(px);       // depth
     if (want_outputs_mask & 2) n += align256(3 * px);   // normal
@@ -44,10 +45,14 @@ void bind_job(DevJob& j, const hp3d_view_desc* d, const Image* ref, const Image*
     p += align256(2 * plane * sizeof(float4));
     float* cost = reinterpret_cast<float*>(p);
     p += align256(2 * plane * sizeof(float));
+    unsigned char* chg = reinterpret_cast<unsigned char*>(p);
+    p += align256(2 * plane);
     j.v.hyp[0] = hyp;
     j.v.hyp[1] = hyp + plane;
     j.v.cost[0] = cost;
     j.v.cost[1] = cost + plane;
+    j.v.chg[0] = chg;
+    j.v.chg[1] = chg + plane;
     const size_t px = (size_t)j.v.W * j.v.H * sizeof(float);
     if (outputs_mask & 1) { j.out_depth = reinterpret_cast<float*>(p); p += align256(px); }
     if (outputs_mask & 2) { j.out_normal = reinterpret_cast<float*>(p); p += align256(3 * px); }

@@ -166,6 +166,7 @@ void hp3d::fill_dev_view(DevView& v, const hp3d_view_desc* d, const Image* ref,
     // patchmatch_mvs.cu:317); planes hold DN, so scale by 255^2
     v.den_eps = ref->den_scale * 1e-6f;
     v.top_k = d->top_k;
+    v.first_iteration = d->first_iteration;
 }
 
 double hp3d::algorithmic_taps(int W, int H, int S, int radius, int stride, int iterations)

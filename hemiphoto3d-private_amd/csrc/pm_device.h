@@ -13,7 +13,8 @@
 //   * hypotheses: two compacted colour planes (red: (x+y) even, black: odd),
 //     each H x Wc with Wc = (W+1)/2.  Pixel (x, y) of colour c sits at
 //     column i = x >> 1 of plane c.  hyp = (depth, nx, ny, nz) as float4,
-//     cost as float.  A half-pass reads the other plane only.
+//     cost as float, and a byte per pixel: changed in its last half-pass.
+//     A half-pass reads the other plane only.
 #pragma once
 #include <stdint.h>
 
@@ -56,6 +57,10 @@ struct DevView {
     int top_k;
     float4* hyp[2];                     // colour planes (depth, nx, ny, nz)
     float* cost[2];
+    // 1 if the pixel's hypothesis changed in its colour's last half-pass;
+    // written by every half-pass, read from iteration first_iteration + 1 on
+    unsigned char* chg[2];
+    int first_iteration;
 };
 
 // Philox4x32-10 (Salmon et al., SC'11).  Used on host and device.

@@ -460,7 +460,9 @@ struct WaveXchg {
 // deterministic function of (pixel, hypothesis), and bc is already at most
 // that cost.  Once propagation has spread planes, 40-60 % of the neighbour
 // candidates are such repeats, but rarely all 64 lanes of a wave at once.  So
-// the wave lists the candidates its lanes still need, any lane evaluates any
+// the wave lists the candidates its lanes still need (also leaving out
+// neighbours unchanged since this pixel's previous pass, see below), any
+// lane evaluates any
 // pixel's candidate (all windows of the workgroup are in LDS), and each
 // owner then applies the costs in L, R, U, D order.  The result is the one
 // the plain per-lane loop gives, with fewer evaluation rounds per wave.
@@ -499,10 +501,18 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
 
     float4 best = make_float4(0.f, 0.f, 0.f, 0.f);
     float bc = 2.0f;
+    bool changed = false;
     unsigned need = 0;   // bit k: neighbour k has to be evaluated
     if (active) {
         best = own[p];
         bc = v.cost[parity][p];
+        // From the second iteration of a solve on, this pixel evaluated its
+        // neighbours' hypotheses one iteration ago; a neighbour that did not
+        // change since (its flag from the other colour's last half-pass)
+        // still holds that hypothesis, whose cost was then >= bc, and bc has
+        // only decreased: it cannot be accepted now.
+        const bool seen = iteration > v.first_iteration;
+        const unsigned char* chg_oth = v.chg[parity ^ 1];
         float4 h[4];
         bool ok[4];
 #pragma unroll
@@ -510,11 +520,14 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
             const bool has = k == 0 ? x >= 1 : k == 1 ? x + 1 < v.W : k == 2 ? y >= 1 : y + 1 < v.H;
             ok[k] = false;
             h[k] = best;
+            bool stale = false;
             if (has) {
-                h[k] = oth[nb(k, p, o)];
+                const int q = nb(k, p, o);
+                h[k] = oth[q];
                 ok[k] = !(h[k].x < dmin || h[k].x > dmax);
+                stale = seen && chg_oth[q] == 0;
             }
-            bool n = ok[k] && !same(h[k], best);
+            bool n = ok[k] && !stale && !same(h[k], best);
 #pragma unroll
             for (int j = 0; j < k; ++j) n = n && !(ok[j] && same(h[j], h[k]));
             need |= (unsigned)n << k;
@@ -573,7 +586,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
     for (int k = 0; k < 4; ++k) {
         if (!((need >> k) & 1u)) continue;
         const float c = xw.cost[slot[k]];
-        if (c < bc) { bc = c; best = oth[nb(k, p, o)]; }
+        if (c < bc) { bc = c; best = oth[nb(k, p, o)]; changed = true; }
     }
 
     double rx, ry, rz;
@@ -599,12 +612,13 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
         if (cnx * ux + cny * uy + cnz * uz > 0.0f) { cnx = -cnx; cny = -cny; cnz = -cnz; }
         const float c = cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x, y, rx, ry, rz, cd, cnx,
                                                  cny, cnz);
-        if (c < bc) { bc = c; best = make_float4(cd, cnx, cny, cnz); }
+        if (c < bc) { bc = c; best = make_float4(cd, cnx, cny, cnz); changed = true; }
         depth_range *= 0.5f;
         normal_range *= 0.5f;
     }
     own[p] = best;
     v.cost[parity][p] = bc;
+    v.chg[parity][p] = changed ? 1 : 0;
 }
 
 // Row-major outputs of pixel (x, y) from the colour planes.

@@ -59,7 +59,7 @@ struct hp3d_pm_ctx {
     std::unordered_map<void*, size_t> user_bufs;   // hp3d_pm_device_alloc
 
     // per-view working set, grown on demand and reused across views
-    hp3d::DevBuf hyp, cost, out_depth, out_normal, out_cost, in_depth, in_normal, staging;
+    hp3d::DevBuf hyp, cost, chg, out_depth, out_normal, out_cost, in_depth, in_normal, staging;
     hp3d::DevBuf seed_pts, seed_owner;
 
     // batched form: state arena, job table, step items, captured graph
@@ -99,7 +99,7 @@ struct hp3d_pm_ctx {
             (void)hipGraphExecDestroy(batch_graph);
             batch_graph = nullptr;
         }
-        for (hp3d::DevBuf* b : {&hyp, &cost, &out_depth, &out_normal, &out_cost, &in_depth,
+        for (hp3d::DevBuf* b : {&hyp, &cost, &chg, &out_depth, &out_normal, &out_cost, &in_depth,
                                 &in_normal, &staging, &seed_pts, &seed_owner, &batch_state,
                                 &batch_jobs, &batch_items})
             release_buf(*b);

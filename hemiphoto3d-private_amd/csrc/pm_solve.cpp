@@ -18,12 +18,17 @@ int bind_state(hp3d_pm_ctx* ctx, DevView& v)
     if (rc) return rc;
     rc = ctx->grow(ctx->cost, 2 * plane * sizeof(float), "cost planes");
     if (rc) return rc;
+    rc = ctx->grow(ctx->chg, 2 * plane, "change flags");
+    if (rc) return rc;
     float4* h = static_cast<float4*>(ctx->hyp.ptr);
     float* c = static_cast<float*>(ctx->cost.ptr);
+    unsigned char* f = static_cast<unsigned char*>(ctx->chg.ptr);
     v.hyp[0] = h;
     v.hyp[1] = h + plane;
     v.cost[0] = c;
     v.cost[1] = c + plane;
+    v.chg[0] = f;
+    v.chg[1] = f + plane;
     return HP3D_OK;
 }
 
