@@ -14,6 +14,10 @@
 
 #include "pm_device.h"
 
+#ifndef HP3D_BUCKET
+#define HP3D_BUCKET 1
+#endif
+
 namespace hp3d {
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
@@ -447,6 +451,9 @@ __device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, 
 struct WaveXchg {
     unsigned char item[256];
     float cost[256];
+#if HP3D_BUCKET
+    int hist[64];
+#endif
 };
 
 // One pixel of a red (parity 0) or black (parity 1) half-pass:
@@ -503,6 +510,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
     float bc = 2.0f;
     bool changed = false;
     unsigned need = 0;   // bit k: neighbour k has to be evaluated
+    float hd[4] = {0.f, 0.f, 0.f, 0.f};   // the candidates' depths
     if (active) {
         best = own[p];
         bc = v.cost[parity][p];
@@ -531,6 +539,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
 #pragma unroll
             for (int j = 0; j < k; ++j) n = n && !(ok[j] && same(h[j], h[k]));
             need |= (unsigned)n << k;
+            hd[k] = h[k].x;
         }
     }
     const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
@@ -550,6 +559,50 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
         base += __popcll(m & below);
         total += (unsigned)__popcll(m);
     }
+#if HP3D_BUCKET
+    if (NT < 9 && iteration == v.first_iteration) {
+        // First iteration: the neighbours hold random, unrelated depths, so
+        // owner order does not cluster the gathers.  Order the items by
+        // inverse depth (disparity) instead, in kBuckets buckets (a counting
+        // sort in LDS; the order within a bucket does not matter), so each
+        // round's windows lie in a narrower band of every source.  (+1.4 %
+        // on 6x6-tap windows; with 11x11 windows it measured slower.)
+        constexpr int kBuckets = 64;
+        const float ilo = 1.0f / dmax, iscale = (float)kBuckets / (1.0f / dmin - ilo);
+        xw.hist[lane] = 0;
+        __builtin_amdgcn_wave_barrier();
+        int bk[4], rk[4];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            bk[k] = 0;
+            rk[k] = 0;
+            if ((need >> k) & 1u) {
+                const int b = (int)((1.0f / hd[k] - ilo) * iscale);
+                bk[k] = b < 0 ? 0 : (b > kBuckets - 1 ? kBuckets - 1 : b);
+                rk[k] = atomicAdd(&xw.hist[bk[k]], 1);
+            }
+        }
+        __builtin_amdgcn_wave_barrier();
+        const int cnt = xw.hist[lane];
+        int incl = cnt;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const int t = __shfl_up(incl, d, 64);
+            if (lane >= d) incl += t;
+        }
+        __builtin_amdgcn_wave_barrier();
+        xw.hist[lane] = incl - cnt;
+        __builtin_amdgcn_wave_barrier();
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            slot[k] = 0;
+            if ((need >> k) & 1u) {
+                slot[k] = xw.hist[bk[k]] + rk[k];
+                xw.item[slot[k]] = (unsigned char)(lane | (k << 6));
+            }
+        }
+    } else
+#endif
     {
         int off = base;
 #pragma unroll
