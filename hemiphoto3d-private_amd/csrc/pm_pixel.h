@@ -447,7 +447,8 @@ __device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, 
 
 // Per-wave LDS scratch of a half-pass: the wave's list of neighbour
 // candidates that must be evaluated (owner lane | candidate << 6), and their
-// costs.  At most 4 per lane.
+// costs, at most 4 per lane; and the depth-bucket counts of the first
+// iteration's ordering.
 struct WaveXchg {
     unsigned char item[256];
     float cost[256];
