@@ -10,7 +10,8 @@ value = steps * W * H * iterations / timed seconds / 1e6   (Mpixel-iters/s)
 
 roofline: the dominant kernel is the half-pass (pm_halfpass_kernel).
   algorithmic work per launch = (W*H/2 active pixels) * 6 candidates * S
-  sources * k taps * 34 FLOP/tap; its average launch duration is measured
+  sources * k taps * 34 FLOP/tap (the reference's work: candidates the kernel
+  skips as provably rejected are counted); its average launch duration is measured
   with HIP events recorded by the library on its own stream during the timed
   steps.  Peak: 157.3 TFLOP/s (MI355X FP32 vector, datasheet).  The
   kernel issues no MFMA and is bound by VALU issue, so bound = "valu".
