@@ -13,8 +13,10 @@
 //     radius-wide halo, clamped at the image edge) in LDS once;
 //   * per candidate and source the plane homography H = A + b (K^-T n)^T / (n.P)
 //     is formed once (window centre in FP64), so a tap costs 3 FMAs for the
-//     warp, one v_rcp_f32, two buffer loads (the texel pairs of two rows) and
-//     6 FMAs for the bilinear sample;
+//     warp, one v_rcp_f32, one buffer load (the row-pair quad) and 6 FMAs
+//     for the bilinear sample;
+//   * neighbour candidates that provably cannot win are skipped and the rest
+//     shared across the wave's lanes (halfpass_pixel in pm_pixel.h);
 //   * the five NCC moments are accumulated in FP32 on values shifted by
 //     -128 DN; the variance cancellation is done in FP64.
 #include <hip/hip_runtime.h>
