@@ -89,9 +89,9 @@ int hp3d::validate_desc(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image**
                          "(radius 1..5, stride 1 or 2)", d->radius, d->stride);
     if (d->top_k < 1 || d->top_k > kMaxTopK)
         return ctx->fail(HP3D_ERR_INVALID_ARG, "top_k %d outside 1..%d", d->top_k, kMaxTopK);
-    if (d->iterations < 0)
-        return ctx->fail(HP3D_ERR_INVALID_ARG, "iterations %d < 0", d->iterations);
-    if (d->first_iteration < 0 || d->first_iteration > (1 << 20))
+    if (d->iterations < 0 || d->iterations > kMaxIterations)
+        return ctx->fail(HP3D_ERR_INVALID_ARG, "iterations %d outside 0..2^20", d->iterations);
+    if (d->first_iteration < 0 || d->first_iteration > kMaxIterations)
         return ctx->fail(HP3D_ERR_INVALID_ARG, "first_iteration %d outside 0..2^20",
                          d->first_iteration);
     if (!(d->depth_min < d->depth_max) || !std::isfinite(d->depth_min) ||

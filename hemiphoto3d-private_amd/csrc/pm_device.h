@@ -27,6 +27,9 @@ constexpr int kMaxTopK = 8;
 constexpr int kMaxSeedRadius = 4096;
 constexpr int kMaxSeedCoord = 1 << 24;
 constexpr float kShiftDN = 128.0f;
+// Bound on iterations and first_iteration of a solve (keeps every step and
+// iteration index, 2 * iterations + 1 included, inside int).
+constexpr int kMaxIterations = 1 << 20;
 // Largest image plane: taps address a plane with 32-bit buffer offsets, and
 // offsets at or above this bound are a guaranteed miss of every plane.
 constexpr unsigned long long kMaxPlaneBytes = 0x7fff0000ull;

@@ -13,6 +13,7 @@
 // and is sub-allocated first-fit, so the loop never calls hipMalloc/hipFree.
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <condition_variable>
 #include <cstring>
 #include <deque>
@@ -382,8 +383,9 @@ int hp3d_server_submit(hp3d_pm_server* s, const hp3d_job_desc* d, uint64_t* id)
     std::lock_guard<std::mutex> lk(s->mu);
     const int need = 2 * s->o.radius + 1;
     if (d->num_src < 1 || d->num_src > kMaxSrc || d->width < need || d->height < need ||
-        !d->ref || !d->out_depth || d->iterations < 0 || d->top_k < 1 || d->top_k > kMaxTopK ||
-        !(d->depth_min < d->depth_max)) {
+        !d->ref || !d->out_depth || d->iterations < 0 || d->iterations > kMaxIterations ||
+        d->top_k < 1 || d->top_k > kMaxTopK || !(d->depth_min < d->depth_max) ||
+        !std::isfinite(d->depth_min) || !std::isfinite(d->depth_max)) {
         s->last_error = "submit: invalid job description";
         s->stats.rejected++;
         return HP3D_ERR_INVALID_ARG;

@@ -80,7 +80,7 @@ typedef struct {
     float depth_min, depth_max;          /* MVSViewParams, patchmatch_mvs.h:23-28 */
     int radius;                          /* window half size 1..5 (.cu:1065-1070) */
     int stride;                          /* window stride 1 or 2 (1 = reference) */
-    int iterations;                      /* PipelineConfig::mvs_iterations */
+    int iterations;                      /* PipelineConfig::mvs_iterations, 0..2^20 */
     int top_k;                           /* NCC_TOP_K = 3 in the reference */
     uint64_t seed;                       /* view seed; reference: 42 + 1e6*ref_id (.cu:1106) */
     const float* init_depth;             /* optional host hypotheses replacing the random */
@@ -89,7 +89,7 @@ typedef struct {
     int num_seeds;                       /*   applied after the init, before the first cost */
     int seed_radius;                     /* half side of each seeded square; reference 50 */
     int first_iteration;                 /* schedule index of the first iteration run: 0, or
-                                            k to resume a run saved after k iterations from
+                                            k <= 2^20 to resume a run saved after k iterations from
                                             its depth and normals (init_depth/init_normal);
                                             the result is then identical to an uninterrupted
                                             run.  hp3d_pm_run_view only. */
@@ -270,7 +270,7 @@ typedef struct {
     double A[HP3D_PM_MAX_SRC][9];
     double b[HP3D_PM_MAX_SRC][3];
     float depth_min, depth_max;
-    int iterations, top_k;
+    int iterations, top_k;                   /* iterations 0..2^20 */
     uint64_t seed;
     float* out_depth;                        /* host, H*W: written when the job completes */
     float* out_normal;                       /* host, H*W*3, nullable */

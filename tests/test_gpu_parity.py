@@ -288,6 +288,13 @@ def test_error_codes(pkg, ctx, c1):
     with pytest.raises(pkg.HP3DError) as e:
         ctx.run_view(bad, job.shape)
     assert e.value.code == pkg.ERR_INVALID_ARG
+    # iteration counts past 2^20 are refused before anything is launched
+    for field in ("iterations", "first_iteration"):
+        bad = desc_for(pkg, job, ref, srcs)
+        setattr(bad, field, (1 << 20) + 1)
+        with pytest.raises(pkg.HP3DError) as e:
+            ctx.run_view(bad, job.shape)
+        assert e.value.code == pkg.ERR_INVALID_ARG, field
     small = pkg.Context(hbm_budget=64 << 10)     # below one 320x240 plane
     with pytest.raises(pkg.HP3DError) as e:
         small.upload(job.ref)

@@ -141,5 +141,9 @@ def test_invalid_submissions(pkg, serve_jobs):
         with pytest.raises(pkg.HP3DError) as e:
             srv.submit(j.ref, j.srcs, j.Kinv, j.A, j.b, 5.0, 5.0, 1)
         assert e.value.code == pkg.ERR_INVALID_ARG
+        for dmax, iters in ((float("inf"), 1), (12.0, (1 << 20) + 1)):
+            with pytest.raises(pkg.HP3DError) as e:
+                srv.submit(j.ref, j.srcs, j.Kinv, j.A, j.b, 2.0, dmax, iters)
+            assert e.value.code == pkg.ERR_INVALID_ARG, (dmax, iters)
         with pytest.raises(pkg.HP3DError):
             srv.poll(123456)
