@@ -3,8 +3,9 @@
 Ragged and tiny images (odd widths, a width that is not a multiple of the
 64-pixel tile, an image exactly the size of the window), every window the
 kernels are built for, 1 and 8 sources (top-3 over fewer than three
-sources), flat images (every window hits the variance guard) and a source
-that sees nothing of the reference (every tap out of bounds).  Gates: random
+sources), flat images (every window hits the variance guard), a source
+that sees nothing of the reference (every tap out of bounds), and the largest
+image plane accepted (2 GiB - 128 KiB) with one row more refused.  Gates: random
 init bit-identical, cost of identical hypotheses within 1e-4 of the FP64
 oracle on every unflagged pixel, and runs that never raise a cost and keep
 depths in range.
@@ -186,3 +187,48 @@ def test_top_k_variants_match_oracle(pkg, orc, ctx, scenes, top_k):
     agree = (np.abs(out["depth"].astype(np.float64) - o64) <= 0.01 * np.abs(o64)).mean()
     print(f"top_k={top_k}: depth within 1 % of the FP64 oracle on {agree:.4f}")
     assert agree >= 0.99
+
+
+def test_largest_plane_accepted_one_row_more_refused(pkg, ctx):
+    """Taps address a plane with 32-bit offsets, so planes are capped 64 KiB
+    below 2 GiB: a 16384-wide FP32 image has an 8-byte-per-pixel row-pair
+    plane of 131072 B per row, which fits at 16383 rows and not at 16384
+    (refused before anything is allocated).  FP16 storage halves the plane,
+    so 16384 rows fit there."""
+    W = 16384
+    big = np.zeros((W, W), np.uint8)            # untouched pages: no resident memory
+    with pytest.raises(pkg.HP3DError) as e:
+        ctx.upload(big, pkg.STORE_F32)
+    assert e.value.code == pkg.ERR_UNSUPPORTED
+    print("16384 x 16384 FP32 refused", flush=True)
+    for rows, store in ((W - 1, pkg.STORE_F32), (W, pkg.STORE_F16)):
+        h = ctx.upload(big[:rows], store)
+        w_, h_, st, nb = ctx.image_info(h)
+        assert (w_, h_, st) == (W, rows, store)
+        assert nb == rows * W * (12 if store == pkg.STORE_F32 else 6)
+        ctx.release(h)
+        print(f"16384 x {rows} store {store} accepted ({nb} B)", flush=True)
+
+
+def test_solve_on_the_largest_plane(pkg, ctx):
+    """A solve over a 16384 x 16383 image, whose FP32 row-pair plane is the
+    largest accepted (2 GiB - 128 KiB): with the source equal to the
+    reference and the identity homography (A = I, b = 0) every tap samples
+    the source at the reference pixel itself, so every window that lies
+    inside the image has NCC 1 -- cost 0 up to FP32 rounding -- for any
+    hypothesis.  A gather addressed wrongly anywhere in the plane (an offset
+    that wraps, a row pitch off by one) breaks that."""
+    W, H, R = 16384, 16383, 5
+    img = np.random.default_rng(7).integers(0, 256, (H, W), dtype=np.uint8)
+    ref = ctx.upload(img, pkg.STORE_F32)
+    f = 16000.0
+    Kinv = np.linalg.inv(np.array([[f, 0, W / 2], [0, f, H / 2], [0, 0, 1.0]]))
+    d = pkg.make_desc(ref, [ref], Kinv, [np.eye(3)], [np.zeros(3)], 2.0, 12.0, radius=R,
+                      stride=2, iterations=1, top_k=1, seed=11)
+    print("solving 16384 x 16383, 1 source, 1 iteration", flush=True)
+    out = ctx.run_view(d, (H, W))
+    ctx.release(ref)
+    inner = out["cost"][R + 1:H - R - 1, R + 1:W - R - 1]
+    print(f"max cost inside {inner.max():.2e}", flush=True)
+    assert inner.max() < 1e-5
+    assert np.all((out["depth"] >= 2.0) & (out["depth"] <= 12.0))
