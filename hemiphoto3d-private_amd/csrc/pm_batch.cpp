@@ -68,7 +68,7 @@ int chosen_variant(const hp3d_pm_ctx* ctx, int radius, int stride, int store, in
     if (!step_variant_tunable(radius, stride)) return 0;
     if (requested >= 0) return requested;
     auto it = ctx->tuned.find(tune_key(radius, stride, store));
-    return it == ctx->tuned.end() ? 0 : it->second;
+    return it == ctx->tuned.end() ? kDefaultStepVariant : it->second;
 }
 
 StepItem make_item(int job, int step, int iterations, uint64_t seed, int W, int H, int variant)

@@ -355,7 +355,9 @@ int hp3d_server_create(const hp3d_server_opts* opts, hp3d_pm_server** out)
     auto s = std::make_unique<hp3d_pm_server>();
     s->o = *opts;
     s->device = opts->device;
-    s->variant = step_variant_tunable(opts->radius, opts->stride) && opts->variant >= 0 ? opts->variant : 0;
+    s->variant = !step_variant_tunable(opts->radius, opts->stride) ? 0
+                 : opts->variant >= 0                              ? opts->variant
+                                                                   : kDefaultStepVariant;
     if (s->variant >= kNumStepVariants) return HP3D_ERR_INVALID_ARG;
     bool ok = hipStreamCreateWithFlags(&s->compute, hipStreamNonBlocking) == hipSuccess &&
               hipStreamCreateWithFlags(&s->upload, hipStreamNonBlocking) == hipSuccess &&

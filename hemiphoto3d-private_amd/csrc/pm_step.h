@@ -43,6 +43,10 @@ extern const int kNumStepVariants;
 // Only radius-5 windows (11x11 at stride 1 or 2) have every variant; other
 // windows run variant 0 (16 x 16 tiles).
 bool step_variant_tunable(int radius, int stride);
+// Variant a radius-5 batch or server runs when none is requested or tuned:
+// 32 x 8 tiles, within 1 % of the fastest shape on mixed-resolution batches
+// at both strides, where 16 x 16 is 8-10 % slower.
+constexpr int kDefaultStepVariant = 1;
 int step_tile_tx(int variant);
 int step_tile_ty(int variant);
 

@@ -218,7 +218,8 @@ int hp3d_pm_fuse(hp3d_pm_ctx* ctx, const hp3d_fuse_view* views, int n, float min
  * bit-identical to hp3d_pm_run_view on the same view.
  * ------------------------------------------------------------------------- */
 typedef struct {
-    int variant;     /* step-kernel variant; -1 = the autotuned choice (or the default) */
+    int variant;     /* step-kernel variant; -1 = the autotuned choice, else variant 1
+                        (32 x 8 tiles); windows other than 11x11 always run variant 0 */
     int use_graph;   /* 1 = capture the step sequence in one hipGraph (default) */
 } hp3d_batch_opts;
 
@@ -259,7 +260,7 @@ typedef struct {
     int max_jobs;         /* concurrent job slots */
     int radius, stride;   /* window shared by every job */
     int store;            /* hp3d_storage of every job's images */
-    int variant;          /* step-kernel variant, -1 = default */
+    int variant;          /* step-kernel variant, -1 = variant 1 (32 x 8 tiles) */
 } hp3d_server_opts;
 
 typedef struct {
