@@ -11,7 +11,14 @@ reference's own FP32 operation order, and either a short
 run within 1 % of the FP64-cost oracle's depth on >= 99 % of pixels or, for
 first iteration 3, a resumed run equal to the uninterrupted one bit for bit.
 Every configuration is printed before it runs.
+
+HP3D_RANDOM_CONFIG_SEEDS=7,8 (a comma-separated list) adds 48 configurations
+from each of those generator seeds, for wider sweeps; two of seed 7's
+(entries 21 and 46, small low-contrast views) exceed the cost bound at 0.31
+and 0.32 times the reference-order FP32 error.
 """
+import os
+
 import numpy as np
 import pytest
 
@@ -38,7 +45,18 @@ def _configs(n=16, seed=20261202):
     return out
 
 
-@pytest.mark.parametrize("cfg", _configs(), ids=lambda c: f"cfg{c['idx']}")
+def _all_configs():
+    out = _configs()
+    for s in filter(None, os.environ.get("HP3D_RANDOM_CONFIG_SEEDS", "").split(",")):
+        out += [dict(c, sweep_seed=int(s)) for c in _configs(48, seed=int(s))]
+    return out
+
+
+def _id(c):
+    return f"s{c['sweep_seed']}-cfg{c['idx']}" if "sweep_seed" in c else f"cfg{c['idx']}"
+
+
+@pytest.mark.parametrize("cfg", _all_configs(), ids=_id)
 def test_random_config_matches_oracle(pkg, orc, ctx, scenes, cfg):
     print(f"config {cfg}", flush=True)
     job = scenes.sphere_job(cfg["W"], cfg["H"], cfg["S"], f=0.9375 * max(cfg["W"], 64),
