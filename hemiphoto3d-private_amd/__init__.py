@@ -89,7 +89,8 @@ class ViewOut(ctypes.Structure):
 
 
 class BatchOpts(ctypes.Structure):
-    _fields_ = [("variant", ctypes.c_int), ("use_graph", ctypes.c_int)]
+    _fields_ = [("variant", ctypes.c_int), ("use_graph", ctypes.c_int),
+                ("persistent", ctypes.c_int), ("chunk", ctypes.c_int)]
 
 
 NUM_VARIANTS = 6
@@ -181,6 +182,11 @@ SIGNATURES = {
     "hp3d_pm_invert3": (ctypes.c_int, [_f64p, _f64p]),
     "hp3d_pm_run_view": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc),
                                         ctypes.POINTER(ViewOut)]),
+    "hp3d_pm_run_views": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc), ctypes.c_int,
+                                         ctypes.POINTER(ViewOut)]),
+    "hp3d_pm_host_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t,
+                                          ctypes.POINTER(ctypes.c_void_p)]),
+    "hp3d_pm_host_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),
     "hp3d_pm_init_hypotheses": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc),
                                                _f32p, _f32p]),
     "hp3d_pm_cost_eval": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc), _f32p, _f32p,
@@ -423,6 +429,42 @@ class Context:
             res["confidence"] = conf
         return res
 
+    def host_array(self, shape, dtype=np.float32):
+        """A numpy array in page-locked host memory owned by the context
+        (hp3d_pm_host_alloc); valid until host_free or close."""
+        dtype = np.dtype(dtype)
+        nbytes = int(np.prod(shape)) * dtype.itemsize
+        p = ctypes.c_void_p()
+        _check(lib().hp3d_pm_host_alloc(self.h, max(nbytes, 1), ctypes.byref(p)), self.h)
+        buf = (ctypes.c_char * max(nbytes, 1)).from_address(p.value)
+        return np.frombuffer(buf, dtype=dtype, count=int(np.prod(shape))).reshape(shape)
+
+    def host_free(self, arr):
+        _check(lib().hp3d_pm_host_free(self.h, ctypes.c_void_p(arr.ctypes.data)), self.h)
+
+    def host_outputs(self, shape, with_normal=True, with_cost=True, pinned=False):
+        """Output arrays of one view: depth, normal and cost (pinned or pageable)."""
+        H, W = shape
+        mk = (lambda shp: self.host_array(shp)) if pinned else \
+            (lambda shp: np.zeros(shp, dtype=np.float32))
+        return {"depth": mk((H, W)), "normal": mk((H, W, 3)) if with_normal else None,
+                "cost": mk((H, W)) if with_cost else None}
+
+    def run_views(self, descs, outs):
+        """Solve views back to back (hp3d_pm_run_views): each view's outputs
+        download while the next view computes.  outs: dicts from
+        host_outputs, filled in place; returns outs."""
+        n = len(descs)
+        darr = (ViewDesc * n)(*descs)
+        oarr = (ViewOut * n)()
+        for i, o in enumerate(outs):
+            oarr[i] = ViewOut(_p(o["depth"], _f32p),
+                              _p(o["normal"], _f32p) if o.get("normal") is not None else None,
+                              _p(o["cost"], _f32p) if o.get("cost") is not None else None,
+                              None, 0)
+        _check(lib().hp3d_pm_run_views(self.h, darr, n, oarr), self.h)
+        return outs
+
     def device_alloc(self, nbytes):
         """Device buffer owned by the context (counted against its HBM budget)."""
         p = ctypes.c_void_p()
@@ -450,8 +492,9 @@ class Context:
         _check(lib().hp3d_pm_run_view(self.h, ctypes.byref(desc), ctypes.byref(out)), self.h)
 
     def batch_run(self, descs, shapes, variant=-1, use_graph=True, with_normal=True,
-                  with_cost=True):
-        """Solve many views in one graph launch; returns one output dict per view."""
+                  with_cost=True, persistent=False, chunk=0):
+        """Solve many views in one graph launch (a graph of step launches, or
+        one persistent kernel); returns one output dict per view."""
         n = len(descs)
         arr = (ViewDesc * n)(*descs)
         outs = (ViewOut * n)()
@@ -464,11 +507,12 @@ class Context:
                               _p(r["normal"], _f32p) if with_normal else None,
                               _p(r["cost"], _f32p) if with_cost else None, None, 0)
             res.append(r)
-        o = BatchOpts(int(variant), 1 if use_graph else 0)
+        o = BatchOpts(int(variant), 1 if use_graph else 0, 1 if persistent else 0, int(chunk))
         _check(lib().hp3d_pm_batch_run(self.h, arr, n, outs, ctypes.byref(o)), self.h)
         return res
 
-    def batch_run_device(self, descs, out_ptrs, variant=-1, use_graph=True):
+    def batch_run_device(self, descs, out_ptrs, variant=-1, use_graph=True, persistent=False,
+                         chunk=0):
         """Batch into caller device buffers: out_ptrs = [(depth, normal, cost)] per view."""
         n = len(descs)
         arr = (ViewDesc * n)(*descs)
@@ -476,7 +520,7 @@ class Context:
         for j, (dd, dn, dc) in enumerate(out_ptrs):
             outs[j] = ViewOut(ctypes.cast(dd, _f32p), ctypes.cast(dn or None, _f32p),
                               ctypes.cast(dc or None, _f32p), None, 1)
-        o = BatchOpts(int(variant), 1 if use_graph else 0)
+        o = BatchOpts(int(variant), 1 if use_graph else 0, 1 if persistent else 0, int(chunk))
         _check(lib().hp3d_pm_batch_run(self.h, arr, n, outs, ctypes.byref(o)), self.h)
 
     def geom_filter(self, maps, depth_tol=0.12, normal_tol_deg=180.0, min_consistent=2):

@@ -88,6 +88,7 @@ StepItem make_item(int job, int step, int iterations, uint64_t seed, int W, int 
         it.pass_seed = pass_seed(seed, it.iteration, it.parity);
     }
     it.emit = step == 2 * iterations;   // the view's last step
+    it.step = step;
     return it;
 }
 
@@ -118,6 +119,36 @@ Plan build_plan(const std::vector<DevJob>& jobs, const hp3d_view_desc* descs, in
         p.count.push_back((int)p.items.size() - off);
         p.tiles.push_back(tiles);
     }
+    return p;
+}
+
+// Every step of the given views in one list for the persistent kernel:
+// items in step order, tile_begin counting tiles across all steps, each
+// view's tile counters at done_base.  *counters: tiles over all views.
+Plan build_flat_plan(const std::vector<DevJob>& jobs, const hp3d_view_desc* descs, int n,
+                     int variant, int max_steps, int* counters, int* units)
+{
+    Plan p;
+    std::vector<int> base(n, 0);
+    int total = 0;
+    for (int j = 0; j < n; ++j) {
+        base[j] = total;
+        total += make_item(j, 0, descs[j].iterations, descs[j].seed, jobs[j].v.W, jobs[j].v.H,
+                           variant).tiles;
+    }
+    int u = 0;
+    for (int s = 0; s < max_steps; ++s)
+        for (int j = 0; j < n; ++j) {
+            const int iters = descs[j].iterations;
+            if (s > 2 * iters) continue;
+            StepItem it = make_item(j, s, iters, descs[j].seed, jobs[j].v.W, jobs[j].v.H, variant);
+            it.tile_begin = u;
+            it.done_base = base[j];
+            u += it.tiles;
+            p.items.push_back(it);
+        }
+    *counters = total;
+    *units = u;
     return p;
 }
 
@@ -238,9 +269,48 @@ int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
         taps += algorithmic_taps(jobs[j].v.W, jobs[j].v.H, jobs[j].v.S, radius, stride,
                                  descs[j].iterations);
     }
-    Plan plan = build_plan(jobs, descs, n, variant, 0, max_steps);
+    const bool persistent = opts ? opts->persistent != 0 : false;
+    const int chunk = opts && opts->chunk > 0 ? opts->chunk : kDefaultPersistChunk;
+    int counters = 0, units = 0;
+    Plan plan;
+    if (persistent) {
+        // the persistent kernel addresses state planes with 31-bit byte offsets
+        for (int j = 0; j < n; ++j)
+            if ((size_t)jobs[j].v.H * jobs[j].v.Wc * sizeof(float4) >= 0x7fffffffull)
+                return ctx->fail(HP3D_ERR_UNSUPPORTED, "batch view %d: %dx%d too large for the "
+                                 "persistent kernel (state plane of 2 GiB)", j, jobs[j].v.W,
+                                 jobs[j].v.H);
+        plan = build_flat_plan(jobs, descs, n, variant, max_steps, &counters, &units);
+        rc = ctx->grow(ctx->batch_done, (size_t)(counters + kCtlWords) * sizeof(int),
+                       "persistent tile counters");
+        if (rc) return rc;
+    } else {
+        plan = build_plan(jobs, descs, n, variant, 0, max_steps);
+    }
     rc = upload_items(ctx, plan);
     if (rc) return rc;
+    int* done = static_cast<int*>(ctx->batch_done.ptr);
+    unsigned long long wait_ticks = 0;
+    if (persistent) {
+        int khz = 0;
+        if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, ctx->device) != hipSuccess ||
+            khz <= 0)
+            khz = 100000;
+        wait_ticks = (unsigned long long)khz * 1000ull * kPersistWaitSeconds;
+    }
+    // the whole batch as launches on the context stream: the step kernels,
+    // or the counter reset and the one persistent kernel
+    auto launch_all = [&]() -> hipError_t {
+        if (!persistent) return launch_plan(ctx, plan, store, radius, stride, variant);
+        hipError_t le = zero_ints(done, counters + kCtlWords, ctx->stream);
+        if (le == hipSuccess)
+            le = launch_persistent(store, radius, stride, variant,
+                                   static_cast<const DevJob*>(ctx->batch_jobs.ptr),
+                                   static_cast<const StepItem*>(ctx->batch_items.ptr),
+                                   (int)plan.items.size(), units, done, counters, chunk,
+                                   wait_ticks, ctx->stream);
+        return le;
+    };
 
     ctx->stats = hp3d_pm_stats{};
     ctx->stats.taps = taps;
@@ -255,8 +325,9 @@ int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
         };
         put(&ctx->batch_jobs.ptr, sizeof(void*));
         put(&ctx->batch_items.ptr, sizeof(void*));
-        int meta[4] = {store, radius, stride, variant};
+        int meta[7] = {store, radius, stride, variant, persistent ? 1 : 0, chunk, counters};
         put(meta, sizeof meta);
+        put(&ctx->batch_done.ptr, sizeof(void*));
         put(plan.offset.data(), plan.offset.size() * sizeof(int));
         put(plan.count.data(), plan.count.size() * sizeof(int));
         put(plan.tiles.data(), plan.tiles.size() * sizeof(int));
@@ -268,7 +339,7 @@ int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
             hipGraph_t g = nullptr;
             e = hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal);
             if (e == hipSuccess) {
-                hipError_t le = launch_plan(ctx, plan, store, radius, stride, variant);
+                hipError_t le = launch_all();
                 e = hipStreamEndCapture(ctx->stream, &g);
                 if (le != hipSuccess) e = le;
             }
@@ -282,7 +353,7 @@ int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
         }
         e = hipGraphLaunch(ctx->batch_graph, ctx->stream);
     } else {
-        e = launch_plan(ctx, plan, store, radius, stride, variant);
+        e = launch_all();
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "batch launch");
     ctx->mark(kPhasePropagate);
@@ -299,9 +370,16 @@ int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
     }
     if (e != hipSuccess) return ctx->hip_fail(e, "batch output copy");
     ctx->mark(kPhaseFinalize);
-    e = hipStreamSynchronize(ctx->stream);
+    int ctl[kCtlWords] = {0, 0};
+    if (persistent)
+        e = hipMemcpyAsync(ctl, done + counters, sizeof ctl, hipMemcpyDeviceToHost, ctx->stream);
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
     ctx->collect_marks(max_steps - 1);
-    return e == hipSuccess ? HP3D_OK : ctx->hip_fail(e, "batch synchronize");
+    if (e != hipSuccess) return ctx->hip_fail(e, "batch synchronize");
+    if (ctl[kCtlAbort])
+        return ctx->fail(HP3D_ERR_DEVICE, "persistent batch: a tile waited more than %d s for "
+                         "its neighbours; outputs are incomplete", kPersistWaitSeconds);
+    return HP3D_OK;
 }
 
 int hp3d_pm_autotune(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n, int* best_variant,

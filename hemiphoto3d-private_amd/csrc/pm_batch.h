@@ -5,6 +5,11 @@
 
 namespace hp3d {
 
+// Persistent batch kernel: units a workgroup takes at a time, and the bound
+// on one dependency wait.
+constexpr int kDefaultPersistChunk = 8;
+constexpr int kPersistWaitSeconds = 10;
+
 // Bytes of one view's state planes plus the staged outputs named by mask
 // (1 depth, 2 normal, 4 cost, 8 confidence).
 size_t job_state_bytes(int W, int H, int want_outputs_mask);

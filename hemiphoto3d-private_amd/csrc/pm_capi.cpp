@@ -8,6 +8,7 @@
 // synchronises after every launch, src/utils/cuda_utils.cuh:18-22).
 #include "pm_runtime.h"
 
+#include <array>
 #include <cmath>
 #include <cstdarg>
 #include <cstdio>
@@ -302,10 +303,12 @@ int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int W, int H, size_
     img.H = H;
     img.store = store;
     img.den_scale = 255.0f * 255.0f;
+    // taps address a plane with 32-bit buffer offsets and 24-bit row offsets
+    if (!image_dims_supported(W, H, store))
+        return ctx->fail(HP3D_ERR_UNSUPPORTED,
+                         "image %dx%d: plane above 2 GiB - 64 KiB or row pitch of 2^24 elements",
+                         W, H);
     image_plane_sizes(W, H, store, &img.pitch_plain, &img.plain_bytes, &img.coef_bytes);
-    // taps address a plane with 32-bit buffer offsets
-    if (img.plain_bytes > kMaxPlaneBytes || img.coef_bytes > kMaxPlaneBytes)
-        return ctx->fail(HP3D_ERR_UNSUPPORTED, "image %dx%d: plane above 2 GiB - 64 KiB", W, H);
     int rc = ctx->dev_alloc(&img.plain, img.plain_bytes, "image plane");
     if (rc) return rc;
     rc = ctx->dev_alloc(&img.coef, img.coef_bytes, "image row-pair plane");
@@ -420,6 +423,54 @@ int hp3d_pm_run_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const hp3d_view_
     if (rc) return rc;
     (void)hipSetDevice(ctx->device);
     return solve_view(ctx, d, ref, srcs, out);
+}
+
+int hp3d_pm_run_views(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, int n, const hp3d_view_out* out)
+{
+    if (!ctx) return HP3D_ERR_INVALID_ARG;
+    if (!d || !out || n < 1) return ctx->fail(HP3D_ERR_INVALID_ARG, "run_views: no views");
+    std::vector<const Image*> refs(n, nullptr);
+    std::vector<std::array<const Image*, kMaxSrc>> srcs(n);
+    for (int i = 0; i < n; ++i) {
+        if (!out[i].depth)
+            return ctx->fail(HP3D_ERR_INVALID_ARG, "run_views: view %d output depth pointer is NULL", i);
+        srcs[i].fill(nullptr);
+        int rc = validate_desc(ctx, &d[i], &refs[i], srcs[i].data());
+        if (rc) return ctx->fail(rc, "run_views view %d: %s", i, ctx->last_error.c_str());
+    }
+    std::vector<const Image* const*> sp(n);
+    for (int i = 0; i < n; ++i) sp[i] = srcs[i].data();
+    (void)hipSetDevice(ctx->device);
+    return solve_views(ctx, n, d, refs.data(), sp.data(), out);
+}
+
+int hp3d_pm_host_alloc(hp3d_pm_ctx* ctx, size_t bytes, void** ptr)
+{
+    if (!ctx) return HP3D_ERR_INVALID_ARG;
+    if (!ptr || bytes == 0) return ctx->fail(HP3D_ERR_INVALID_ARG, "host_alloc: bad arguments");
+    *ptr = nullptr;
+    (void)hipSetDevice(ctx->device);
+    hipError_t e = hipHostMalloc(ptr, bytes, hipHostMallocDefault);
+    if (e != hipSuccess) {
+        *ptr = nullptr;
+        return ctx->hip_fail(e, "host_alloc");
+    }
+    ctx->host_bufs[*ptr] = bytes;
+    return HP3D_OK;
+}
+
+int hp3d_pm_host_free(hp3d_pm_ctx* ctx, void* ptr)
+{
+    if (!ctx) return HP3D_ERR_INVALID_ARG;
+    auto it = ctx->host_bufs.find(ptr);
+    if (it == ctx->host_bufs.end())
+        return ctx->fail(HP3D_ERR_NOT_FOUND, "host_free: %p was not allocated by this context", ptr);
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->copy_stream) (void)hipStreamSynchronize(ctx->copy_stream);
+    (void)hipHostFree(ptr);
+    ctx->host_bufs.erase(it);
+    return HP3D_OK;
 }
 
 int hp3d_pm_init_hypotheses(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, float* depth,

@@ -33,6 +33,9 @@ constexpr int kMaxIterations = 1 << 20;
 // Largest image plane: taps address a plane with 32-bit buffer offsets, and
 // offsets at or above this bound are a guaranteed miss of every plane.
 constexpr unsigned long long kMaxPlaneBytes = 0x7fff0000ull;
+// Source taps form row offsets with a 24-bit multiply (iy * pitch), so a
+// plane's row pitch (in elements) must stay below 2^24.
+constexpr int kMaxRowPitch = 1 << 24;
 
 enum Store : int { kStoreF32 = 0, kStoreF16 = 1 };
 

@@ -108,9 +108,18 @@ extern "C" int hp3d_pm_geom_filter(hp3d_pm_ctx* ctx, hp3d_geom_view* views, int 
         if (v.width < 1 || v.height < 1 || !v.depth || !v.normal || v.num_src < 0 ||
             v.num_src > kMaxSrc)
             return ctx->fail(HP3D_ERR_INVALID_ARG, "geom_filter: view %d is malformed", i);
-        for (int s = 0; s < v.num_src; ++s)
+        // a view's own map is filtered in place while its sources are read,
+        // so it must not be one of them; a repeated source would count twice
+        for (int s = 0; s < v.num_src; ++s) {
             if (v.src[s] < 0 || v.src[s] >= n)
                 return ctx->fail(HP3D_ERR_INVALID_ARG, "geom_filter: view %d source %d out of range", i, s);
+            if (v.src[s] == i)
+                return ctx->fail(HP3D_ERR_INVALID_ARG, "geom_filter: view %d lists itself as a source", i);
+            for (int r = 0; r < s; ++r)
+                if (v.src[r] == v.src[s])
+                    return ctx->fail(HP3D_ERR_INVALID_ARG, "geom_filter: view %d lists source %d twice",
+                                     i, v.src[s]);
+        }
     }
     (void)hipSetDevice(ctx->device);
     // one device block: per view depth, normal, confidence

@@ -96,6 +96,60 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, int b
     return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, kBufferRsrcWord3);
 }
 
+// Access to the hypothesis state planes (hyp, cost, change flags).
+// COH = false: ordinary loads and stores (one kernel launch per pass: the
+// launch boundary orders passes).  COH = true: device-coherent accesses
+// (the sc1 cache policy, bypassing the per-CU L1 and the per-XCD L2 caches
+// that are not coherent across XCDs), for the persistent batch kernel, where
+// a tile's pass reads what workgroups on other XCDs wrote in the same launch.
+// The image planes are read-only within a launch and keep ordinary cached
+// loads either way.  Planes addressed this way hold less than 2 GiB.
+template <bool COH> struct StateIO;
+template <> struct StateIO<false> {
+    static __device__ __forceinline__ float4 ld4(const float4* p, int i) { return p[i]; }
+    static __device__ __forceinline__ float ld1(const float* p, int i) { return p[i]; }
+    static __device__ __forceinline__ unsigned ldb(const unsigned char* p, int i) { return p[i]; }
+    static __device__ __forceinline__ void st4(float4* p, int i, float4 x) { p[i] = x; }
+    static __device__ __forceinline__ void st1(float* p, int i, float x) { p[i] = x; }
+    static __device__ __forceinline__ void stb(unsigned char* p, int i, unsigned x)
+    {
+        p[i] = (unsigned char)x;
+    }
+};
+template <> struct StateIO<true> {
+    static constexpr int kSC1 = 16;   // cache-policy operand: sc1 (device scope)
+    static __device__ __forceinline__ __amdgpu_buffer_rsrc_t rs(const void* p)
+    {
+        return make_rsrc(p, 0x7fffffff);
+    }
+    static __device__ __forceinline__ float4 ld4(const float4* p, int i)
+    {
+        return __builtin_bit_cast(float4,
+                                  __builtin_amdgcn_raw_buffer_load_b128(rs(p), i * 16, 0, kSC1));
+    }
+    static __device__ __forceinline__ float ld1(const float* p, int i)
+    {
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs(p), i * 4, 0, kSC1));
+    }
+    static __device__ __forceinline__ unsigned ldb(const unsigned char* p, int i)
+    {
+        return __builtin_amdgcn_raw_buffer_load_b8(rs(p), i, 0, kSC1);
+    }
+    static __device__ __forceinline__ void st4(float4* p, int i, float4 x)
+    {
+        __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, x),
+                                               rs(p), i * 16, 0, kSC1);
+    }
+    static __device__ __forceinline__ void st1(float* p, int i, float x)
+    {
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), rs(p), i * 4, 0, kSC1);
+    }
+    static __device__ __forceinline__ void stb(unsigned char* p, int i, unsigned x)
+    {
+        __builtin_amdgcn_raw_buffer_store_b8((unsigned char)x, rs(p), i, 0, kSC1);
+    }
+};
+
 // a (1 - t) + b t as two FMAs.
 __device__ __forceinline__ float lerp_fma(float t, float a, float b)
 {
@@ -182,6 +236,15 @@ __device__ __forceinline__ WinRef window_ref_sums(const float* __restrict__ lwin
 // it takes the reference sums from wr, which window_ref_sums forms with the
 // same additions in the same order as the masked loop does when every tap is
 // valid, so a window's sums do not depend on which variant its wave ran.
+//
+// The source moments are taken of s - s0, with s0 the window's first valid
+// sample (the NCC is invariant to a constant shift of the samples).  Unshifted,
+// a low-contrast window's samples sit far from zero (up to 128 DN against a
+// spread of a few DN), and the FP32 rounding of s^2 and of the running sums
+// carries over into n*sum(s^2) - sum(s)^2 amplified by mean^2 / variance: up
+// to 1.2e-3 of cost error on small views.  Shifted, the summands are of the
+// size of the spread.  When every tap is valid both loops take the first
+// tap's sample, so the variants still agree bit for bit.
 template <int NT, int ST, int STORE, bool MASKED>
 __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
                                            const float* __restrict__ lwin, int lp, float cx,
@@ -191,6 +254,8 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
 {
     constexpr int R = (NT - 1) * ST / 2;
     float sr = 0.f, ss = 0.f, srr = 0.f, sss = 0.f, srs = 0.f;
+    float s0 = 0.f;     // the shift: first valid sample of the window
+    bool have = false;  // (masked loop) a valid tap has been seen
     int cnt = 0;
     // window rows: unrolled by two for the 11-tap (stride 1) rows, where the
     // extra loads in flight pay; one at a time for the 6-tap rows, where
@@ -217,11 +282,20 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
             // invalid tap's load goes nowhere and reads zeros (so sval = 0)
             const bool ok = !MASKED || ((fabsf(wz) >= 1e-8f) & ((unsigned)ix <= umax) &
                                         ((unsigned)iy <= vmax));
-            const float sval = img.sample(ix, iy, au, aw, ok);
+            const float sraw = img.sample(ix, iy, au, aw, ok);
             float r = lrow[i * ST];
+            float sval;
             if (MASKED) {
+                // an invalid tap read zeros; before the first valid tap s0
+                // stays 0, at it s0 becomes that tap's sample
+                s0 = have ? s0 : sraw;
+                have = have || ok;
+                sval = ok ? sraw - s0 : 0.0f;
                 r = ok ? r : 0.0f;
                 cnt += ok ? 1 : 0;
+            } else {
+                if (i == 0) s0 = j == 0 ? sraw : s0;
+                sval = sraw - s0;
             }
             if (MASKED) {
                 sr += r;
@@ -424,7 +498,7 @@ __device__ __forceinline__ float4 init_hyp(const DevView& v, int x, int y, uint6
 
 // Cost of the current hypothesis of the colour-`parity` pixel of lane (tx, ty)
 // in tile (bx, by); returns false if the lane has no pixel.
-template <int NT, int ST, int STORE, int TX, int TY>
+template <int NT, int ST, int STORE, int TX, int TY, bool COH = false>
 __device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, int bx, int by,
                                            int tx, int ty, int parity)
 {
@@ -438,10 +512,11 @@ __device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, 
     double rx, ry, rz;
     ray_fp64(v.kinv_d, x, y, rx, ry, rz);
     const int p = y * v.Wc + i;
-    const float4 h = v.hyp[parity][p];
+    const float4 h = StateIO<COH>::ld4(v.hyp[parity], p);
     const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
-    v.cost[parity][p] =
-        cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x, y, rx, ry, rz, h.x, h.y, h.z, h.w);
+    StateIO<COH>::st1(v.cost[parity], p,
+                      cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x, y, rx, ry, rz, h.x, h.y,
+                                               h.z, h.w));
     return true;
 }
 
@@ -475,13 +550,14 @@ struct WaveXchg {
 // owner then applies the costs in L, R, U, D order.  The result is the one
 // the plain per-lane loop gives, with fewer evaluation rounds per wave.
 // Every thread of the workgroup must call this (it synchronises the group).
-template <int NT, int ST, int STORE, int TX, int TY>
+template <int NT, int ST, int STORE, int TX, int TY, bool COH = false>
 __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* tile, int bx, int by,
                                                int tx, int ty, int iteration, int parity,
                                                uint64_t pass_seed, WaveXchg& xw)
 {
     constexpr int R = (NT - 1) * ST / 2;
     using G = TileGeom<R, STORE, TX, TY>;
+    using IO = StateIO<COH>;
     const int lane = threadIdx.x & 63;
     const int wave0 = threadIdx.x & ~63;
     const int i = bx * TX + tx;
@@ -513,8 +589,8 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
     unsigned need = 0;   // bit k: neighbour k has to be evaluated
     float hd[4] = {0.f, 0.f, 0.f, 0.f};   // the candidates' depths
     if (active) {
-        best = own[p];
-        bc = v.cost[parity][p];
+        best = IO::ld4(own, p);
+        bc = IO::ld1(v.cost[parity], p);
         // From the second iteration of a solve on, this pixel evaluated its
         // neighbours' hypotheses one iteration ago; a neighbour that did not
         // change since (its flag from the other colour's last half-pass)
@@ -532,9 +608,9 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
             bool stale = false;
             if (has) {
                 const int q = nb(k, p, o);
-                h[k] = oth[q];
+                h[k] = IO::ld4(oth, q);
                 ok[k] = !(h[k].x < dmin || h[k].x > dmax);
-                stale = seen && chg_oth[q] == 0;
+                stale = seen && IO::ldb(chg_oth, q) == 0;
             }
             bool n = ok[k] && !stale && !same(h[k], best);
 #pragma unroll
@@ -626,7 +702,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
             const int tx_o = t_o % TX, ty_o = t_o / TX;
             const int i_o = bx * TX + tx_o, y_o = by * TY + ty_o;
             const int o_o = (y_o + parity) & 1, x_o = 2 * i_o + o_o;
-            const float4 h = oth[nb(k, y_o * Wc + i_o, o_o)];
+            const float4 h = IO::ld4(oth, nb(k, y_o * Wc + i_o, o_o));
             double rx, ry, rz;
             ray_fp64(v.kinv_d, x_o, y_o, rx, ry, rz);
             xw.cost[g] = cand_cost<NT, ST, STORE>(v, tile + ty_o * G::LP + (x_o - 2 * bx * TX),
@@ -640,7 +716,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
     for (int k = 0; k < 4; ++k) {
         if (!((need >> k) & 1u)) continue;
         const float c = xw.cost[slot[k]];
-        if (c < bc) { bc = c; best = oth[nb(k, p, o)]; changed = true; }
+        if (c < bc) { bc = c; best = IO::ld4(oth, nb(k, p, o)); changed = true; }
     }
 
     double rx, ry, rz;
@@ -670,22 +746,23 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
         depth_range *= 0.5f;
         normal_range *= 0.5f;
     }
-    own[p] = best;
-    v.cost[parity][p] = bc;
-    v.chg[parity][p] = changed ? 1 : 0;
+    IO::st4(own, p, best);
+    IO::st1(v.cost[parity], p, bc);
+    IO::stb(v.chg[parity], p, changed ? 1u : 0u);
 }
 
 // Row-major outputs of pixel (x, y) from the colour planes.
+template <bool COH = false>
 __device__ __forceinline__ void emit_pixel(const DevView& v, int x, int y, float* depth,
                                            float* normal, float* cost, float* conf)
 {
     const int c = (x + y) & 1, p = y * v.Wc + (x >> 1);
     const size_t q = (size_t)y * v.W + x;
-    const float4 h = v.hyp[c][p];
+    const float4 h = StateIO<COH>::ld4(v.hyp[c], p);
     if (depth) depth[q] = h.x;
     if (normal) { normal[3 * q] = h.y; normal[3 * q + 1] = h.z; normal[3 * q + 2] = h.w; }
     if (cost || conf) {
-        const float k = v.cost[c][p];
+        const float k = StateIO<COH>::ld1(v.cost[c], p);
         if (cost) cost[q] = k;
         if (conf) conf[q] = 1.0f - k * 0.5f;   // patchmatch_mvs.cu:1249-1252
     }

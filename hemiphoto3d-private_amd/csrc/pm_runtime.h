@@ -33,6 +33,17 @@ inline void image_plane_sizes(int W, int H, int store, int* pitch, size_t* plain
     *coef_bytes = (size_t)*pitch * H * (store == HP3D_STORE_F32 ? 8 : 4);   // row pairs
 }
 
+// True if a W x H image with this storage can be addressed by the source
+// taps: row pitch below 2^24 elements and both planes at most kMaxPlaneBytes.
+inline bool image_dims_supported(int W, int H, int store)
+{
+    if (W < 1 || H < 1 || W > kMaxRowPitch - 64) return false;
+    int pitch;
+    size_t plain_bytes, coef_bytes;
+    image_plane_sizes(W, H, store, &pitch, &plain_bytes, &coef_bytes);
+    return pitch < kMaxRowPitch && plain_bytes <= kMaxPlaneBytes && coef_bytes <= kMaxPlaneBytes;
+}
+
 struct DevBuf {
     void* ptr = nullptr;
     size_t bytes = 0;
@@ -57,13 +68,21 @@ struct hp3d_pm_ctx {
     std::unordered_map<hp3d_img, hp3d::Image> images;
     hp3d_img next_handle = 0;
     std::unordered_map<void*, size_t> user_bufs;   // hp3d_pm_device_alloc
+    std::unordered_map<void*, size_t> host_bufs;   // hp3d_pm_host_alloc (pinned)
+
+    // multi-view pipeline (hp3d_pm_run_views): a second stream for result
+    // downloads, two output staging slots, and per slot the events "finalized
+    // into the slot" and "slot downloaded"
+    hipStream_t copy_stream = nullptr;
+    hp3d::DevBuf out_slot[2];
+    hipEvent_t ev_fin[2] = {nullptr, nullptr}, ev_copied[2] = {nullptr, nullptr};
 
     // per-view working set, grown on demand and reused across views
     hp3d::DevBuf hyp, cost, chg, out_depth, out_normal, out_cost, in_depth, in_normal, staging;
     hp3d::DevBuf seed_pts, seed_owner;
 
     // batched form: state arena, job table, step items, captured graph
-    hp3d::DevBuf batch_state, batch_jobs, batch_items;
+    hp3d::DevBuf batch_state, batch_jobs, batch_items, batch_done;
     hipGraphExec_t batch_graph = nullptr;
     std::vector<unsigned char> batch_graph_key;
     std::map<int, int> tuned;   // (radius, stride, store) key -> variant
@@ -95,13 +114,26 @@ struct hp3d_pm_ctx {
         images.clear();
         for (auto& kv : user_bufs) dev_free(kv.first, kv.second);
         user_bufs.clear();
+        for (auto& kv : host_bufs) (void)hipHostFree(kv.first);
+        host_bufs.clear();
+        if (copy_stream) {
+            (void)hipStreamSynchronize(copy_stream);
+            (void)hipStreamDestroy(copy_stream);
+            copy_stream = nullptr;
+        }
+        for (int k = 0; k < 2; ++k) {
+            release_buf(out_slot[k]);
+            if (ev_fin[k]) (void)hipEventDestroy(ev_fin[k]);
+            if (ev_copied[k]) (void)hipEventDestroy(ev_copied[k]);
+            ev_fin[k] = ev_copied[k] = nullptr;
+        }
         if (batch_graph) {
             (void)hipGraphExecDestroy(batch_graph);
             batch_graph = nullptr;
         }
         for (hp3d::DevBuf* b : {&hyp, &cost, &chg, &out_depth, &out_normal, &out_cost, &in_depth,
                                 &in_normal, &staging, &seed_pts, &seed_owner, &batch_state,
-                                &batch_jobs, &batch_items})
+                                &batch_jobs, &batch_items, &batch_done})
             release_buf(*b);
     }
 };
@@ -116,6 +148,10 @@ uint64_t pass_seed(uint64_t seed, int iteration, int parity);
 
 int solve_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
                const Image* const* srcs, const hp3d_view_out* out);
+// Views solved back to back; host outputs of view i are downloaded on the
+// copy stream while view i + 1 computes.  srcs[i] holds view i's sources.
+int solve_views(hp3d_pm_ctx* ctx, int n, const hp3d_view_desc* d, const Image* const* refs,
+                const Image* const* const* srcs, const hp3d_view_out* out);
 int init_only(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
               const Image* const* srcs, float* depth, float* normal);
 int cost_only(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,

@@ -83,7 +83,8 @@ struct SJob {
     hp3d_job_desc d{};
     std::vector<uint8_t> host_img;       // (1+S) images, W*H each
     hp3d_job_status st{};
-    size_t footprint = 0, off = 0;
+    size_t footprint = 0, off = 0;       // footprint: bytes the job needs once admitted
+    bool holds_block = false;            // [off, off + footprint) is this job's arena block
     int slot = -1;
     int step = 0;                        // next step to run
     bool ready = false, finished = false, cancel = false;
@@ -185,9 +186,11 @@ void hp3d_pm_server::finish_job(const std::shared_ptr<SJob>& j, int state, int e
         free_slots.push_back(j->slot);
         j->slot = -1;
     }
-    if (j->footprint) {
+    // only an admitted job holds an arena block: a job cancelled while
+    // queued has a footprint but no block, and must release nothing
+    if (j->holds_block) {
         arena.release(j->off, j->footprint);
-        j->footprint = 0;
+        j->holds_block = false;
     }
     if (j->ready_ev) {
         (void)hipEventDestroy(j->ready_ev);
@@ -289,6 +292,7 @@ void hp3d_pm_server::loop()
                 if (!arena.alloc(j->footprint, &off)) break;
                 queue.pop_front();
                 j->off = off;
+                j->holds_block = true;
                 j->slot = free_slots.back();
                 free_slots.pop_back();
                 layout(*j);
@@ -401,12 +405,14 @@ int hp3d_server_submit(hp3d_pm_server* s, const hp3d_job_desc* d, uint64_t* id)
     auto j = std::make_shared<SJob>();
     j->d = *d;
     j->footprint = s->footprint(*d);
-    // taps address an image plane with 32-bit buffer offsets
-    int pitch;
-    size_t plain_bytes, coef_bytes;
-    image_plane_sizes(d->width, d->height, s->o.store, &pitch, &plain_bytes, &coef_bytes);
-    if (j->footprint > s->arena.size() || plain_bytes > kMaxPlaneBytes ||
-        coef_bytes > kMaxPlaneBytes) {
+    // taps address an image plane with 32-bit buffer offsets and 24-bit row
+    // offsets
+    if (!image_dims_supported(d->width, d->height, s->o.store)) {
+        s->last_error = "submit: image plane above 2 GiB - 64 KiB or row pitch of 2^24 elements";
+        s->stats.rejected++;
+        return HP3D_ERR_UNSUPPORTED;
+    }
+    if (j->footprint > s->arena.size()) {
         s->last_error = "submit: job footprint exceeds the server's HBM budget";
         s->stats.rejected++;
         return HP3D_ERR_OOM_BUDGET;

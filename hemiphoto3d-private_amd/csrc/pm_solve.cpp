@@ -2,6 +2,7 @@
 //
 // init -> cost(red) -> cost(black) -> iterations x {half-pass red, black}
 // -> finalize, all on the context stream, one synchronisation at the end.
+#include <algorithm>
 #include <cstring>
 
 #include "pm_runtime.h"
@@ -116,16 +117,14 @@ int finish(hp3d_pm_ctx* ctx, int n_prop)
 
 }  // namespace
 
-int solve_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
-               const Image* const* srcs, const hp3d_view_out* out)
+namespace {
+
+// Everything of one view up to its last half-pass, on the context stream:
+// init (or caller hypotheses), seeds, initial cost, the iterations.
+int enqueue_passes(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref, DevView& v,
+                   int* n_prop)
 {
-    DevView v;
-    fill_dev_view(v, d, ref, srcs);
-    int rc = bind_state(ctx, v);
-    if (rc) return rc;
-    ctx->stats = hp3d_pm_stats{};
-    ctx->stats.taps = algorithmic_taps(v.W, v.H, v.S, d->radius, d->stride, d->iterations);
-    ctx->mark(kPhaseStart);
+    int rc = HP3D_OK;
     hipError_t e = hipSuccess;
     if (d->init_depth) {
         rc = import_hypotheses(ctx, v, d->init_depth, d->init_normal);
@@ -141,20 +140,155 @@ int solve_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
         e = launch_pass(v, ref->store, d->radius, d->stride, kPassCost, 0, parity, 0, ctx->stream);
     if (e != hipSuccess) return ctx->hip_fail(e, "cost launch");
     ctx->mark(kPhaseCost);
-    int n_prop = 0;
     for (int it = d->first_iteration; it < d->first_iteration + d->iterations; ++it)
         for (int parity = 0; parity < 2; ++parity) {
             e = launch_pass(v, ref->store, d->radius, d->stride, kPassPropagate, it, parity,
                             pass_seed(d->seed, it, parity), ctx->stream);
             if (e != hipSuccess) return ctx->hip_fail(e, "half-pass launch");
             ctx->mark(kPhasePropagate);
-            ++n_prop;
+            ++*n_prop;
         }
+    return HP3D_OK;
+}
+
+// Bytes of one view's staged host outputs: depth, normal, cost, confidence.
+size_t staged_bytes(const hp3d_view_desc* d, const Image* ref, const hp3d_view_out* o)
+{
+    if (o->on_device) return 0;
+    const size_t n = (size_t)ref->W * ref->H * sizeof(float);
+    (void)d;
+    return (o->depth ? n : 0) + (o->normal ? 3 * n : 0) + (o->cost ? n : 0) +
+           (o->confidence ? n : 0);
+}
+
+}  // namespace
+
+int solve_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
+               const Image* const* srcs, const hp3d_view_out* out)
+{
+    DevView v;
+    fill_dev_view(v, d, ref, srcs);
+    int rc = bind_state(ctx, v);
+    if (rc) return rc;
+    ctx->stats = hp3d_pm_stats{};
+    ctx->stats.taps = algorithmic_taps(v.W, v.H, v.S, d->radius, d->stride, d->iterations);
+    ctx->mark(kPhaseStart);
+    int n_prop = 0;
+    rc = enqueue_passes(ctx, d, ref, v, &n_prop);
+    if (rc) return rc;
     rc = emit_outputs(ctx, v, out->depth, out->normal, out->cost, out->confidence,
                       out->on_device != 0);
     if (rc) return rc;
     ctx->mark(kPhaseFinalize);
     return finish(ctx, n_prop);
+}
+
+int solve_views(hp3d_pm_ctx* ctx, int n, const hp3d_view_desc* d, const Image* const* refs,
+                const Image* const* const* srcs, const hp3d_view_out* out)
+{
+    // the pipeline's stream, events and two staging slots (sized for the
+    // largest view before the loop, so no buffer is reallocated while a
+    // download may still read it)
+    hipError_t e = hipSuccess;
+    if (!ctx->copy_stream)
+        e = hipStreamCreateWithFlags(&ctx->copy_stream, hipStreamNonBlocking);
+    for (int k = 0; k < 2 && e == hipSuccess; ++k) {
+        if (!ctx->ev_fin[k]) e = hipEventCreateWithFlags(&ctx->ev_fin[k], hipEventDisableTiming);
+        if (e == hipSuccess && !ctx->ev_copied[k])
+            e = hipEventCreateWithFlags(&ctx->ev_copied[k], hipEventDisableTiming);
+    }
+    if (e != hipSuccess) return ctx->hip_fail(e, "pipeline set-up");
+    size_t slot_bytes = 0;
+    for (int i = 0; i < n; ++i) slot_bytes = std::max(slot_bytes, staged_bytes(&d[i], refs[i], &out[i]));
+    e = hipStreamSynchronize(ctx->copy_stream);
+    if (e != hipSuccess) return ctx->hip_fail(e, "copy stream");
+    for (int k = 0; k < 2 && slot_bytes; ++k) {
+        int rc = ctx->grow(ctx->out_slot[k], slot_bytes, "output staging slot");
+        if (rc) return rc;
+    }
+    // state planes for the largest view up front: growing them inside the
+    // loop would free planes that queued work still uses
+    {
+        DevView big;
+        std::memset(&big, 0, sizeof big);
+        size_t best = 0;
+        for (int i = 0; i < n; ++i) {
+            const size_t plane = (size_t)refs[i]->H * ((refs[i]->W + 1) / 2);
+            if (plane > best) {
+                best = plane;
+                big.H = refs[i]->H;
+                big.Wc = (refs[i]->W + 1) / 2;
+            }
+        }
+        int rc = bind_state(ctx, big);
+        if (rc) return rc;
+    }
+    ctx->stats = hp3d_pm_stats{};
+    ctx->mark(kPhaseStart);
+    int n_prop = 0;
+    bool slot_used[2] = {false, false};
+    // staged output pointers of the view in each slot
+    float* sd[2][4] = {};
+    int rc = HP3D_OK;
+    for (int i = 0; i <= n && rc == HP3D_OK; ++i) {
+        if (i < n) {
+            DevView v;
+            fill_dev_view(v, &d[i], refs[i], srcs[i]);
+            rc = bind_state(ctx, v);
+            if (rc) break;
+            ctx->stats.taps += algorithmic_taps(v.W, v.H, v.S, d[i].radius, d[i].stride,
+                                                d[i].iterations);
+            rc = enqueue_passes(ctx, &d[i], refs[i], v, &n_prop);
+            if (rc) break;
+            const hp3d_view_out& o = out[i];
+            if (o.on_device) {
+                e = launch_finalize(v, o.depth, o.normal, o.cost, o.confidence, ctx->stream);
+            } else {
+                const int k = i & 1;
+                // the slot's previous view must be downloaded before it is overwritten
+                if (slot_used[k]) e = hipStreamWaitEvent(ctx->stream, ctx->ev_copied[k], 0);
+                float* p = static_cast<float*>(ctx->out_slot[k].ptr);
+                const size_t px = (size_t)v.W * v.H;
+                float* dst[4] = {o.depth, o.normal, o.cost, o.confidence};
+                const size_t cnt[4] = {px, 3 * px, px, px};
+                for (int q = 0; q < 4; ++q) {
+                    sd[k][q] = dst[q] ? p : nullptr;
+                    if (dst[q]) p += cnt[q];
+                }
+                if (e == hipSuccess)
+                    e = launch_finalize(v, sd[k][0], sd[k][1], sd[k][2], sd[k][3], ctx->stream);
+                if (e == hipSuccess) e = hipEventRecord(ctx->ev_fin[k], ctx->stream);
+                slot_used[k] = true;
+            }
+            if (e != hipSuccess) { rc = ctx->hip_fail(e, "finalize"); break; }
+            ctx->mark(kPhaseFinalize);
+        }
+        // download the previous view while this one computes; enqueued after
+        // this view's launches, so a copy that blocks the host (pageable
+        // destination) still overlaps them
+        const int j = i - 1;
+        if (j >= 0 && !out[j].on_device) {
+            const int k = j & 1;
+            const size_t px = (size_t)refs[j]->W * refs[j]->H;
+            float* dst[4] = {out[j].depth, out[j].normal, out[j].cost, out[j].confidence};
+            const size_t cnt[4] = {px, 3 * px, px, px};
+            e = hipStreamWaitEvent(ctx->copy_stream, ctx->ev_fin[k], 0);
+            for (int q = 0; q < 4 && e == hipSuccess; ++q)
+                if (dst[q])
+                    e = hipMemcpyAsync(dst[q], sd[k][q], cnt[q] * sizeof(float),
+                                       hipMemcpyDeviceToHost, ctx->copy_stream);
+            if (e == hipSuccess) e = hipEventRecord(ctx->ev_copied[k], ctx->copy_stream);
+            if (e != hipSuccess) rc = ctx->hip_fail(e, "output download");
+        }
+    }
+    // drain both streams whatever happened, so no copy outlives the call
+    hipError_t e1 = hipStreamSynchronize(ctx->stream);
+    hipError_t e2 = hipStreamSynchronize(ctx->copy_stream);
+    ctx->collect_marks(n_prop);
+    if (rc) return rc;
+    if (e1 != hipSuccess) return ctx->hip_fail(e1, "stream synchronize");
+    if (e2 != hipSuccess) return ctx->hip_fail(e2, "copy stream synchronize");
+    return HP3D_OK;
 }
 
 int init_only(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,

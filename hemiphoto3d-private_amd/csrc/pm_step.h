@@ -31,7 +31,12 @@ struct StepItem {
     int tiles_x;       // tiles per row of the view
     int tiles;         // tiles of the view
     uint64_t pass_seed;
+    int step;          // the view's step index: 0 start, 1 + half-pass index
+    int done_base;     // persistent kernel: first of the view's tile counters
 };
+
+// Persistent batch kernel: control words after the per-tile counters.
+enum PersistCtl : int { kCtlNextChunk = 0, kCtlAbort = 1, kCtlWords = 2 };
 
 struct StepVariant {
     int tx, ty;        // compacted columns x rows per workgroup
@@ -52,5 +57,17 @@ int step_tile_ty(int variant);
 
 hipError_t launch_step(int store, int radius, int stride, int variant, const DevJob* jobs,
                        const StepItem* items, int n_items, int total_tiles, hipStream_t st);
+
+// One launch for a whole batch: items of every step in step order, their
+// tile_begin counting units (tiles) across all steps; done holds one counter
+// per (view, tile), followed by kCtlWords control words, all zero at launch
+// (zero_ints).  chunk: consecutive units a workgroup takes at a time.
+// wait_ticks: bound, in wall-clock ticks, on any one dependency wait; a wait
+// that exceeds it sets done[n_counters + kCtlAbort] and every workgroup exits.
+hipError_t launch_persistent(int store, int radius, int stride, int variant, const DevJob* jobs,
+                             const StepItem* items, int n_items, int total_units, int* done,
+                             int n_counters, int chunk, unsigned long long wait_ticks,
+                             hipStream_t st);
+hipError_t zero_ints(int* p, int n, hipStream_t st);
 
 }  // namespace hp3d

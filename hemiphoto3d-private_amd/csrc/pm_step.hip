@@ -20,6 +20,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <algorithm>
+
 #include "pm_device.h"
 #include "pm_pixel.h"
 #include "pm_step.h"
@@ -79,6 +81,228 @@ void pm_step_kernel(const DevJob* __restrict__ jobs, const StepItem* __restrict_
                                     job.out_conf);
         }
     }
+}
+
+// ---------------------------------------------------------------------------
+// Persistent batch kernel: one launch runs every step of every view.
+//
+// The units (view, step, tile) are listed step by step, so every unit a
+// unit depends on comes earlier in the list.  A workgroup takes the next
+// `chunk` consecutive units from one device counter and runs them in order.
+// A half-pass of tile T at step s reads the other colour of T and of its
+// left, right, upper and lower neighbour tiles (the L, R, U, D candidates of
+// its edge pixels) as step s - 1 left them, and overwrites its own colour,
+// which step s - 1 of those same tiles read: so it waits until the five
+// tiles' counters (steps completed) reach s, and afterwards sets T's counter
+// to s + 1.  A start unit (random init and initial cost of both colours)
+// depends on nothing.
+//
+// Progress: take the first unit of the list that is not finished.  Every
+// unit before it is finished, so its chunk has been taken (chunks are
+// taken in list order by running workgroups) and its holder has reached
+// it; its dependencies come earlier in the list, so its wait ends.  This
+// holds for any grid size and needs no co-residency.  Each wait is still
+// bounded in time; a wait that exceeds the bound sets the abort word, every
+// workgroup then leaves, and the host reports the failure.
+//
+// Memory: the state planes are read and written with device-coherent (sc1)
+// accesses (StateIO<true>), which bypass the per-CU L1 and per-XCD L2 caches
+// that are not coherent across XCDs; before a tile's counter is set, every
+// wave waits for its state stores to complete (vmcnt 0), and the counter
+// itself is a device-scope store that waiting workgroups poll with
+// device-scope loads.  A workgroup reads the planes only after its poll has
+// seen the counters, so it reads what the producer stored.  The image planes
+// are read-only during the launch and keep cached loads.
+// ---------------------------------------------------------------------------
+template <int NT, int ST, int STORE, int TX, int TY, int WPE>
+__global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
+void pm_persist_kernel(const DevJob* __restrict__ jobs, const StepItem* __restrict__ items,
+                       int n_items, int total_units, int* done, int n_counters, int chunk,
+                       unsigned long long wait_ticks)
+{
+    constexpr int R = (NT - 1) * ST / 2;
+    using G = TileGeom<R, STORE, TX, TY>;
+    __shared__ float tile[G::LH * G::LP];
+    __shared__ WaveXchg xw[TX * TY / 64];
+    __shared__ int s_word;
+    int* const ctl = done + n_counters;
+    const int tx = threadIdx.x % TX, ty = threadIdx.x / TX;
+
+    for (;;) {
+        if (threadIdx.x == 0)
+            s_word = __hip_atomic_load(&ctl[kCtlAbort], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                         ? total_units
+                         : __hip_atomic_fetch_add(&ctl[kCtlNextChunk], 1, __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT) * chunk;
+        __syncthreads();
+        const int u0 = __builtin_amdgcn_readfirstlane(s_word);
+        __syncthreads();
+        if (u0 >= total_units) return;
+        const int u1 = min(u0 + chunk, total_units);
+        for (int u = u0; u < u1; ++u) {
+            int lo = 0, hi = n_items - 1;     // last item with tile_begin <= u
+            while (lo < hi) {
+                const int mid = (lo + hi + 1) >> 1;
+                if (items[mid].tile_begin <= u) lo = mid;
+                else hi = mid - 1;
+            }
+            const StepItem& it = items[lo];
+            const DevJob& job = jobs[it.job];
+            const DevView& v = job.v;
+            const int lt = u - it.tile_begin;
+            const int by = lt / it.tiles_x, bx = lt - by * it.tiles_x;
+            // the reference tile depends only on the images: staged before the wait
+            stage_ref_tile<R, STORE, TX, TY>(v, tile, bx, by);
+            if (it.mode != kStepStart) {
+                if (threadIdx.x == 0) {
+                    const int* cnt = done + it.done_base;
+                    const int ty_n = it.tiles / it.tiles_x;
+                    int dep[5] = {lt, -1, -1, -1, -1};
+                    if (bx > 0) dep[1] = lt - 1;
+                    if (bx + 1 < it.tiles_x) dep[2] = lt + 1;
+                    if (by > 0) dep[3] = lt - it.tiles_x;
+                    if (by + 1 < ty_n) dep[4] = lt + it.tiles_x;
+                    int abort = 0;
+                    const unsigned long long t0 = wall_clock64();
+                    for (int k = 0; k < 5 && !abort; ++k) {
+                        if (dep[k] < 0) continue;
+                        while (__hip_atomic_load(&cnt[dep[k]], __ATOMIC_RELAXED,
+                                                 __HIP_MEMORY_SCOPE_AGENT) < it.step) {
+                            __builtin_amdgcn_s_sleep(2);
+                            if (__hip_atomic_load(&ctl[kCtlAbort], __ATOMIC_RELAXED,
+                                                  __HIP_MEMORY_SCOPE_AGENT)) {
+                                abort = 1;
+                                break;
+                            }
+                            if (wall_clock64() - t0 > wait_ticks) {
+                                __hip_atomic_store(&ctl[kCtlAbort], 1, __ATOMIC_RELAXED,
+                                                   __HIP_MEMORY_SCOPE_AGENT);
+                                abort = 1;
+                                break;
+                            }
+                        }
+                    }
+                    s_word = abort;
+                }
+                __syncthreads();
+                const int abort = __builtin_amdgcn_readfirstlane(s_word);
+                __syncthreads();
+                if (abort) return;
+            }
+            const int i = bx * TX + tx, y = by * TY + ty;
+            if (it.mode == kStepStart) {
+                if (y < v.H) {
+                    for (int c = 0; c < 2; ++c) {
+                        const int x = 2 * i + ((y + c) & 1);
+                        if (x < v.W) StateIO<true>::st4(v.hyp[c], y * v.Wc + i, init_hyp(v, x, y, job.seed));
+                    }
+                }
+                cost_pixel<NT, ST, STORE, TX, TY, true>(v, tile, bx, by, tx, ty, 0);
+                cost_pixel<NT, ST, STORE, TX, TY, true>(v, tile, bx, by, tx, ty, 1);
+            } else {
+                halfpass_pixel<NT, ST, STORE, TX, TY, true>(v, tile, bx, by, tx, ty, it.iteration,
+                                                            it.parity, it.pass_seed,
+                                                            xw[threadIdx.x >> 6]);
+            }
+            if (it.emit && y < v.H) {
+                for (int c = 0; c < 2; ++c) {
+                    const int x = 2 * i + ((y + c) & 1);
+                    if (x < v.W)
+                        emit_pixel<true>(v, x, y, job.out_depth, job.out_normal, job.out_cost,
+                                         job.out_conf);
+                }
+            }
+            // every wave's state stores complete before the tile's counter moves
+            __builtin_amdgcn_s_waitcnt(0);
+            __syncthreads();
+            if (threadIdx.x == 0)
+                __hip_atomic_store(&done[it.done_base + lt], it.step + 1, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+}
+
+__global__ void zero_ints_kernel(int* p, int n)
+{
+    for (int k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) p[k] = 0;
+}
+
+hipError_t zero_ints(int* p, int n, hipStream_t st)
+{
+    const int blocks = std::min((n + 255) / 256, 1024);
+    hipLaunchKernelGGL(zero_ints_kernel, dim3(std::max(blocks, 1)), dim3(256), 0, st, p, n);
+    return hipGetLastError();
+}
+
+template <int NT, int ST, int STORE, int TX, int TY, int WPE>
+static hipError_t launch_persist_t(const DevJob* jobs, const StepItem* items, int n_items,
+                                   int total_units, int* done, int n_counters, int chunk,
+                                   unsigned long long wait_ticks, hipStream_t st)
+{
+    // as many workgroups as fit the GPU at once (more would only start after
+    // the work is gone); computed once per kernel instance
+    static int grid = 0;
+    if (grid == 0) {
+        int dev = 0, cus = 0, per_cu = 0;
+        hipError_t e = hipGetDevice(&dev);
+        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+        if (e == hipSuccess)
+            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                &per_cu, reinterpret_cast<const void*>(&pm_persist_kernel<NT, ST, STORE, TX, TY, WPE>),
+                TX * TY, 0);
+        if (e != hipSuccess) return e;
+        grid = std::max(1, cus * std::max(per_cu, 1));
+    }
+    hipLaunchKernelGGL((pm_persist_kernel<NT, ST, STORE, TX, TY, WPE>), dim3(grid), dim3(TX * TY),
+                       0, st, jobs, items, n_items, total_units, done, n_counters, chunk,
+                       wait_ticks);
+    return hipGetLastError();
+}
+
+template <int NT, int ST, int STORE>
+static hipError_t launch_persist_tuned(int variant, const DevJob* jobs, const StepItem* items,
+                                       int n_items, int total_units, int* done, int n_counters,
+                                       int chunk, unsigned long long wait_ticks, hipStream_t st)
+{
+#define HP3D_PV(V_, TX_, TY_)                                                                     \
+    case V_:                                                                                    \
+        return launch_persist_t<NT, ST, STORE, TX_, TY_, 1>(jobs, items, n_items, total_units,  \
+                                                            done, n_counters, chunk, wait_ticks, st);
+    switch (variant) {
+        HP3D_PV(0, 16, 16) HP3D_PV(1, 32, 8) HP3D_PV(2, 8, 32) HP3D_PV(3, 64, 4)
+        HP3D_PV(4, 32, 4) HP3D_PV(5, 16, 8)
+        default: return hipErrorInvalidValue;
+    }
+#undef HP3D_PV
+}
+
+hipError_t launch_persistent(int store, int radius, int stride, int variant, const DevJob* jobs,
+                             const StepItem* items, int n_items, int total_units, int* done,
+                             int n_counters, int chunk, unsigned long long wait_ticks,
+                             hipStream_t st)
+{
+    if (variant < 0 || variant >= kNumStepVariants || chunk < 1) return hipErrorInvalidValue;
+    if (!step_variant_tunable(radius, stride) && variant != 0) return hipErrorInvalidValue;
+#define HP3D_PT(R_, S_, ST_)                                                                   \
+    if (radius == R_ && stride == S_ && store == ST_)                                         \
+        return launch_persist_tuned<(2 * R_) / S_ + 1, S_, ST_>(variant, jobs, items, n_items,  \
+                                                               total_units, done, n_counters,  \
+                                                               chunk, wait_ticks, st);
+#define HP3D_PP(R_, S_, ST_)                                                                   \
+    if (radius == R_ && stride == S_ && store == ST_)                                         \
+        return launch_persist_t<(2 * R_) / S_ + 1, S_, ST_, 16, 16, 1>(                        \
+            jobs, items, n_items, total_units, done, n_counters, chunk, wait_ticks, st);
+    HP3D_PT(5, 1, kStoreF32) HP3D_PT(5, 2, kStoreF32)
+    HP3D_PT(5, 1, kStoreF16) HP3D_PT(5, 2, kStoreF16)
+    HP3D_PP(1, 1, kStoreF32) HP3D_PP(2, 1, kStoreF32) HP3D_PP(3, 1, kStoreF32)
+    HP3D_PP(4, 1, kStoreF32) HP3D_PP(1, 2, kStoreF32) HP3D_PP(2, 2, kStoreF32)
+    HP3D_PP(3, 2, kStoreF32) HP3D_PP(4, 2, kStoreF32)
+    HP3D_PP(1, 1, kStoreF16) HP3D_PP(2, 1, kStoreF16) HP3D_PP(3, 1, kStoreF16)
+    HP3D_PP(4, 1, kStoreF16) HP3D_PP(1, 2, kStoreF16) HP3D_PP(2, 2, kStoreF16)
+    HP3D_PP(3, 2, kStoreF16) HP3D_PP(4, 2, kStoreF16)
+#undef HP3D_PT
+#undef HP3D_PP
+    return hipErrorInvalidValue;
 }
 
 // ---------------------------------------------------------------------------

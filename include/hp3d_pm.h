@@ -147,6 +147,25 @@ int hp3d_pm_invert3(const double M[9], double Minv[9]);
  * desc->iterations red/black half-passes.  Blocks until out is written. */
 int hp3d_pm_run_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, const hp3d_view_out* out);
 
+/* Solve n views back to back, as the reference's per-view loop in
+ * runPatchMatchMVS does (patchmatch_mvs.cu:1076-1258, one view after the
+ * other, each result downloaded before the next view starts, :1243-1252).
+ * Here the host outputs of view i are downloaded on a second stream while
+ * view i + 1 computes, so the downloads overlap the solves; outputs are
+ * identical to n hp3d_pm_run_view calls.  out[i] as for hp3d_pm_run_view
+ * (host or device pointers per view); host outputs in memory from
+ * hp3d_pm_host_alloc download at full link speed.  Blocks until every
+ * output is written. */
+int hp3d_pm_run_views(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, int n,
+                      const hp3d_view_out* out);
+
+/* Page-locked host memory owned by the context, for output buffers that
+ * downloads write at full link speed (the reference returns cv::Mat in
+ * pageable memory, patchmatch_mvs.cu:1243-1252).  Freed by hp3d_pm_host_free
+ * or when the context is destroyed. */
+int hp3d_pm_host_alloc(hp3d_pm_ctx* ctx, size_t bytes, void** ptr);
+int hp3d_pm_host_free(hp3d_pm_ctx* ctx, void* ptr);
+
 /* Random initial hypotheses only (the reference's mvs_iterations = 0 maps,
  * minus the cost).  depth H*W, normal H*W*3, host memory. */
 int hp3d_pm_init_hypotheses(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, float* depth,
@@ -220,7 +239,12 @@ int hp3d_pm_fuse(hp3d_pm_ctx* ctx, const hp3d_fuse_view* views, int n, float min
 typedef struct {
     int variant;     /* step-kernel variant; -1 = the autotuned choice, else variant 1
                         (32 x 8 tiles); windows other than 11x11 always run variant 0 */
-    int use_graph;   /* 1 = capture the step sequence in one hipGraph (default) */
+    int use_graph;   /* 1 = capture the launches in one hipGraph (default) */
+    int persistent;  /* 1 = one persistent kernel for the whole batch: a device work list
+                        of (view, step, tile) units, each half-pass of a tile starting as
+                        soon as that tile and its 4 neighbours finished the previous pass
+                        (per-tile counters), instead of one launch per step */
+    int chunk;       /* persistent: units a workgroup takes at a time (0 = 8) */
 } hp3d_batch_opts;
 
 int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,

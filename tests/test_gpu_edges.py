@@ -147,16 +147,13 @@ def test_top_k_variants_match_oracle(pkg, orc, ctx, scenes, top_k):
     sources: the kernels keep a 3-entry list when at most 3 costs are
     averaged and an 8-entry one otherwise.  The selection and averaging are
     what this checks, on a scene chosen to be hard for FP32 moments (7x7
-    windows of low contrast far from mid-gray): there a single source's
-    cost differs from FP64 by up to 2.4e-4, all of it from accumulating the
-    moments in FP32 (a numpy restatement of the kernel's arithmetic with
-    FP64 moments agrees to 3e-6), and costs averaged over k sources carry
-    much of it.  So the tolerance here is 4e-4 -- against ~3e-3 for the
-    reference's own FP32 operation order, which the GPU must also beat 4x
-    on a sample; the 1e-4 gate of the workloads is tested in
-    test_gpu_parity.py, test_gpu_precision.py and above.  A 2-iteration run
-    must put >= 99 % of pixels within 1 % of the oracle's depth."""
-    tol = 4e-4
+    windows of low contrast far from mid-gray, where unshifted FP32 moments
+    put a single source's cost 2.4e-4 off FP64): with one or two sources
+    averaged nothing dilutes a source's error, so this holds the kernels'
+    shifted moments to the same 1e-4 gate as the workloads, and to a tenth
+    of the reference's own FP32 operation order on a sample.  A 2-iteration
+    run must put >= 99 % of pixels within 1 % of the oracle's depth."""
+    tol = COST_TOL
     job = scenes.sphere_job(160, 120, 8, radius=3, stride=1, iterations=2, scene_seed=31,
                             name="topk")
     v = oracle_view(orc, job, top_k=top_k)
@@ -179,7 +176,7 @@ def test_top_k_variants_match_oracle(pkg, orc, ctx, scenes, top_k):
               f"{c[wy, wx]:.7f} fp64 {c64[wy, wx]:.7f}, per-source fp64 {np.round(per_src, 6)}; "
               f"reference-order FP32 on a sample {ref32:.2e}")
         assert err[ok].max() <= tol, (top_k, store)
-        assert err[ys[pick], xs[pick]].max() <= 0.25 * ref32
+        assert err[ys[pick], xs[pick]].max() <= 0.1 * ref32
         out = ctx.run_view(d, job.shape)
         for h in handles:
             ctx.release(h)
@@ -232,3 +229,30 @@ def test_solve_on_the_largest_plane(pkg, ctx):
     print(f"max cost inside {inner.max():.2e}", flush=True)
     assert inner.max() < 1e-5
     assert np.all((out["depth"] >= 2.0) & (out["depth"] <= 12.0))
+
+
+def test_widest_row_pitch_accepted_and_solved(pkg, ctx):
+    """Source taps form row offsets with a 24-bit multiply, so a plane's row
+    pitch (the width rounded up to 64 elements) must stay below 2^24: width
+    2^24 - 64 is accepted, one pixel more is refused before anything is
+    allocated.  On the widest image, with the source equal to the reference
+    and the identity homography, every window inside the image must cost 0
+    (up to FP32 rounding): a row offset that wrapped would break that."""
+    W, H, R = (1 << 24) - 64, 12, 3
+    wide = np.zeros((H, W + 1), np.uint8)
+    with pytest.raises(pkg.HP3DError) as e:
+        ctx.upload(wide, pkg.STORE_F16)
+    assert e.value.code == pkg.ERR_UNSUPPORTED
+    del wide
+    img = np.random.default_rng(3).integers(0, 256, (H, W), dtype=np.uint8)
+    ref = ctx.upload(img, pkg.STORE_F16)
+    f = float(W)
+    Kinv = np.linalg.inv(np.array([[f, 0, W / 2], [0, f, H / 2], [0, 0, 1.0]]))
+    d = pkg.make_desc(ref, [ref], Kinv, [np.eye(3)], [np.zeros(3)], 2.0, 12.0, radius=R,
+                      stride=2, iterations=1, top_k=1, seed=5)
+    print(f"solving {W} x {H}, 1 source, 1 iteration", flush=True)
+    out = ctx.run_view(d, (H, W), with_normal=False)
+    ctx.release(ref)
+    inner = out["cost"][R:H - R - 1, R:W - R - 1]
+    print(f"max cost inside {inner.max():.2e}", flush=True)
+    assert inner.max() < 1e-5

@@ -76,3 +76,18 @@ def test_filter_order_dependence(pkg, orc, ctx, view_set):
     ctx.geom_filter(rev)
     for g, r in zip(rev, expect):
         assert np.array_equal(g["depth"], r["depth"])
+
+
+def test_filter_refuses_self_and_repeated_sources(pkg, ctx, view_set):
+    """A view filtered in place cannot also be read as its own source, and a
+    source listed twice would be counted twice: both are refused before any
+    map is touched."""
+    for bad in ([0, 1], [1, 1]):
+        maps = [{k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in m.items()}
+                for m in view_set[:4]]
+        maps[0]["src"] = bad
+        before = maps[0]["depth"].copy()
+        with pytest.raises(pkg.HP3DError) as e:
+            ctx.geom_filter(maps)
+        assert e.value.code == pkg.ERR_INVALID_ARG, bad
+        assert np.array_equal(maps[0]["depth"], before)

@@ -1,21 +1,19 @@
 """Seeded random configurations of the plain path against the oracle (MI355X).
 
-Sixteen small views drawn from one fixed RNG: size (odd and even, down to a
-few windows across), 1..8 sources at random arc angles, window radius 1..5,
-stride 1 or 2, top-k 1..8, both storages, 1..3 iterations, and a first
-iteration of 0 or 3.  Each must give the oracle's random init bit for bit,
-costs of identical hypotheses within the tolerance the other tests state
-(1e-4, 4e-4 when fewer than three sources are averaged) or, for the tiny
-windows where FP32 rounding is larger, within a quarter of the error of the
-reference's own FP32 operation order, and either a short
-run within 1 % of the FP64-cost oracle's depth on >= 99 % of pixels or, for
-first iteration 3, a resumed run equal to the uninterrupted one bit for bit.
-Every configuration is printed before it runs.
+112 small views from three fixed generator seeds (16 from one, 48 from each
+of two more): size (odd and even, down to a few windows across), 1..8
+sources at random arc angles, window radius 1..5, stride 1 or 2, top-k 1..8,
+both storages, 1..3 iterations, and a first iteration of 0 or 3.  Each must
+give the oracle's random init bit for bit, costs of identical hypotheses
+within 1e-4 of the FP64 oracle on every unflagged pixel, whatever the window
+size and however few sources are averaged (the reference's own FP32
+operation order is printed beside it, on a sample), and either a short run
+within 1 % of the FP64-cost oracle's depth on >= 99 % of pixels or, for first
+iteration 3, a resumed run equal to the uninterrupted one bit for bit.  Every
+configuration is printed before it runs.
 
-HP3D_RANDOM_CONFIG_SEEDS=7,8 (a comma-separated list) adds 48 configurations
-from each of those generator seeds, for wider sweeps; two of seed 7's
-(entries 21 and 46, small low-contrast views) exceed the cost bound at 0.31
-and 0.32 times the reference-order FP32 error.
+HP3D_RANDOM_CONFIG_SEEDS (a comma-separated list) adds 48 configurations from
+each further generator seed, for wider sweeps.
 """
 import os
 
@@ -45,10 +43,14 @@ def _configs(n=16, seed=20261202):
     return out
 
 
+DEFAULT_SWEEP_SEEDS = (7, 8)
+
+
 def _all_configs():
     out = _configs()
-    for s in filter(None, os.environ.get("HP3D_RANDOM_CONFIG_SEEDS", "").split(",")):
-        out += [dict(c, sweep_seed=int(s)) for c in _configs(48, seed=int(s))]
+    extra = [int(s) for s in filter(None, os.environ.get("HP3D_RANDOM_CONFIG_SEEDS", "").split(","))]
+    for s in list(DEFAULT_SWEEP_SEEDS) + [s for s in extra if s not in DEFAULT_SWEEP_SEEDS]:
+        out += [dict(c, sweep_seed=s) for c in _configs(48, seed=s)]
     return out
 
 
@@ -63,8 +65,7 @@ def test_random_config_matches_oracle(pkg, orc, ctx, scenes, cfg):
                             radius=cfg["radius"], stride=cfg["stride"],
                             iterations=cfg["iterations"], angles=cfg["angles"],
                             scene_seed=cfg["scene_seed"], name=f"rand{cfg['idx']}")
-    k = min(cfg["top_k"], cfg["S"])
-    tol = 1e-4 if k >= 3 else 4e-4
+    tol = 1e-4
     v = oracle_view(orc, job, top_k=cfg["top_k"])
     ref = ctx.upload(job.ref, cfg["store"])
     srcs = [ctx.upload(s, cfg["store"]) for s in job.srcs]
@@ -79,15 +80,12 @@ def test_random_config_matches_oracle(pkg, orc, ctx, scenes, cfg):
     ok = flags == 0
     err = np.abs(c.astype(np.float64) - c64)[ok]
     assert ok.mean() > 0.5
-    # windows of 9 taps carry more FP32 rounding than the 121- and 36-tap
-    # windows the 1e-4 gate is stated for: there the bound is a quarter of
-    # what the reference's own FP32 operation order gives (on a sample)
     ys, xs = np.nonzero(ok)
     pick = np.random.default_rng(cfg["idx"]).choice(len(xs), min(len(xs), 1500), replace=False)
     ref32 = max(abs(v.cost_fp32(int(xs[i]), int(ys[i]), float(d0[ys[i], xs[i]]),
                                 n0[ys[i], xs[i]]) - c64[ys[i], xs[i]]) for i in pick)
     print(f"  cost max|err| {err.max():.2e}; reference-order FP32 on a sample {ref32:.2e}")
-    assert err.max() <= max(tol, 0.25 * ref32), float(err.max())
+    assert err.max() <= tol, float(err.max())
     if cfg["first"]:
         # resumed: `first` iterations, then the rest from the saved state with
         # first_iteration set, must equal the uninterrupted run bit for bit

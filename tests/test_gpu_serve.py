@@ -147,3 +147,42 @@ def test_invalid_submissions(pkg, serve_jobs):
             assert e.value.code == pkg.ERR_INVALID_ARG, (dmax, iters)
         with pytest.raises(pkg.HP3DError):
             srv.poll(123456)
+
+
+def test_cancel_queued_job_releases_nothing(pkg, ctx, serve_jobs, expected):
+    """A job cancelled while it waits in the queue owns no arena block, so
+    cancelling it must not free one: here job A runs at arena offset 0, job
+    B (A's size) waits because the budget leaves room only for a smaller
+    job C, B is cancelled and C submitted.  C must be placed beside A, not
+    over it: both results equal their per-view solves and the arena's peak
+    stays inside the budget."""
+    big, small = serve_jobs[2], serve_jobs[0]
+    fp = {}
+    for name, job in (("big", big), ("small", small)):
+        with pkg.Server(256 * MB, max_jobs=1, stride=2) as srv:
+            assert srv.wait(srv.submit_job(job, iterations=1), 60)["state"] == pkg.JOB_DONE
+            fp[name] = srv.stats()["arena_peak"]
+    assert fp["big"] > fp["small"] + 256
+    budget = fp["big"] + fp["small"] + 256
+    # A runs long enough for B and C to be queued and cancelled meanwhile
+    iters_a = 40
+    ref = ctx.upload(big.ref, pkg.STORE_F16)
+    srcs = [ctx.upload(s, pkg.STORE_F16) for s in big.srcs]
+    exp_a = ctx.run_view(pkg.make_desc(ref, srcs, big.Kinv, big.A, big.b, big.dmin, big.dmax,
+                                       radius=5, stride=2, iterations=iters_a, seed=big.seed),
+                         big.shape)
+    for h in [ref] + srcs:
+        ctx.release(h)
+    with pkg.Server(budget, max_jobs=4, stride=2) as srv:
+        a = srv.submit_job(big, iterations=iters_a)
+        b = srv.submit_job(big, seed=12345)
+        srv.cancel(b)
+        c = srv.submit_job(small)
+        assert srv.wait(b, 120)["state"] == pkg.JOB_CANCELLED
+        assert srv.wait(a, 120)["state"] == pkg.JOB_DONE
+        assert srv.wait(c, 120)["state"] == pkg.JOB_DONE
+        _check(srv.result(a), exp_a)
+        _check(srv.result(c), expected[0])
+        s = srv.stats()
+        print("arena", fp, budget, s["arena_peak"])
+        assert s["arena_peak"] <= budget
