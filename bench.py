@@ -63,6 +63,8 @@ def parse():
                     default="plain",
                     help="plain: the headline line; the others print their own JSON line")
     ap.add_argument("--views", type=int, default=48, help="batched: views per batch")
+    ap.add_argument("--chunk", type=int, default=8,
+                    help="batched: units a persistent workgroup takes at a time")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: arrivals per second")
     ap.add_argument("--duration", type=float, default=120.0, help="serving: arrival window (s)")
     ap.add_argument("--stride", type=int, default=None,
@@ -387,12 +389,18 @@ def form_batched(a, entry, pkg, scenes):
         best, times = a.variant, np.full(pkg.NUM_VARIANTS, -1.0)
     else:
         best, times = ctx.autotune(d16)
-    ctx.batch_run_device(d16, outs, variant=best)         # warm-up (captures the graph)
     reps = 2
-    t0 = time.perf_counter()
-    for _ in range(reps):
-        ctx.batch_run_device(d16, outs, variant=best)
-    batched_s = (time.perf_counter() - t0) / reps
+
+    def timed_batch(**kw):
+        ctx.batch_run_device(d16, outs, variant=best, **kw)     # warm-up (captures the graph)
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            ctx.batch_run_device(d16, outs, variant=best, **kw)
+        return (time.perf_counter() - t0) / reps
+    # one persistent kernel for the whole batch, and the graph of step launches
+    persist_s = timed_batch(persistent=True, chunk=a.chunk)
+    steps_s = timed_batch(persistent=False)
+    batched_s = min(persist_s, steps_s)
     # plain kernel one view after the other: FP32 storage, and the batch's storage
     def serial(descs):
         ctx.run_view_device(descs[0], *outs[0])
@@ -422,6 +430,10 @@ def form_batched(a, entry, pkg, scenes):
                "views": a.views, "mpixel_iters": round(mpix, 2), "taps": taps,
                "render_s": round(render_s, 1)},
            "ms_per_batch": round(batched_s * 1e3, 2),
+           "persistent_value": round(mpix / persist_s, 2),
+           "persistent_chunk": a.chunk,
+           "step_graph_value": round(mpix / steps_s, 2),
+           "launch_form": "persistent kernel" if persist_s <= steps_s else "graph of step launches",
            "plain_serial_value": round(mpix / plain_s, 2),
            "speedup_vs_plain_serial": round(plain_s / batched_s, 3),
            "plain_serial_same_storage_value": round(mpix / plain_same_s, 2),

@@ -111,7 +111,8 @@ class JobDesc(ctypes.Structure):
                 ("depth_min", ctypes.c_float), ("depth_max", ctypes.c_float),
                 ("iterations", ctypes.c_int), ("top_k", ctypes.c_int), ("seed", ctypes.c_uint64),
                 ("out_depth", _f32p), ("out_normal", _f32p), ("out_cost", _f32p),
-                ("fail_at_step", ctypes.c_int)]
+                ("fail_at_step", ctypes.c_int), ("ref_image", ctypes.c_uint64),
+                ("src_image", ctypes.c_uint64 * MAX_SRC), ("level", ctypes.c_int)]
 
 
 class JobStatus(ctypes.Structure):
@@ -126,7 +127,9 @@ class ServerStats(ctypes.Structure):
                 ("cancelled", ctypes.c_longlong), ("rejected", ctypes.c_longlong),
                 ("taps", ctypes.c_double), ("mpix_iters", ctypes.c_double),
                 ("arena_bytes", ctypes.c_size_t), ("arena_peak", ctypes.c_size_t),
-                ("max_active", ctypes.c_int)]
+                ("max_active", ctypes.c_int), ("image_uploads", ctypes.c_longlong),
+                ("image_reuses", ctypes.c_longlong), ("resident_images", ctypes.c_longlong),
+                ("wait_ms", ctypes.c_double)]
 
 
 class GeomView(ctypes.Structure):
@@ -219,6 +222,14 @@ SIGNATURES = {
     "hp3d_server_forget": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
     "hp3d_server_get_stats": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ServerStats)]),
     "hp3d_server_last_error": (ctypes.c_char_p, [ctypes.c_void_p]),
+    "hp3d_server_image_add": (ctypes.c_int, [ctypes.c_void_p, _u8p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_size_t, ctypes.c_int,
+                                             ctypes.POINTER(ctypes.c_uint64)]),
+    "hp3d_server_image_remove": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64]),
+    "hp3d_server_image_read": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_uint64, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int),
+                                              ctypes.POINTER(ctypes.c_int), _f32p,
+                                              ctypes.c_size_t]),
     "hp3d_server_destroy": (ctypes.c_int, [ctypes.c_void_p]),
     "hp3d_pm_device_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t,
                                             ctypes.POINTER(ctypes.c_void_p)]),
@@ -750,17 +761,60 @@ class Server:
     def __exit__(self, *exc):
         self.close()
 
+    def image_add(self, image, levels=1):
+        """Keep an (H, W) uint8 image resident (with `levels` pyramid levels);
+        returns its handle for submit(ref=handle, srcs=[handles])."""
+        a = np.ascontiguousarray(image, dtype=np.uint8)
+        H, W = a.shape
+        h = ctypes.c_uint64(0)
+        code = lib().hp3d_server_image_add(self.h, _p(a, _u8p), W, H, W, int(levels),
+                                           ctypes.byref(h))
+        if code != OK:
+            raise HP3DError(code, lib().hp3d_server_last_error(self.h).decode())
+        return h.value
+
+    def image_remove(self, handle):
+        _check(lib().hp3d_server_image_remove(self.h, handle))
+
+    def image_read(self, handle, level=0):
+        """A resident image's pyramid level as DN values, (H_l, W_l) float32."""
+        W, H = ctypes.c_int(), ctypes.c_int()
+        _check(lib().hp3d_server_image_read(self.h, handle, level, ctypes.byref(W),
+                                            ctypes.byref(H), None, 0))
+        out = np.zeros((H.value, W.value), np.float32)
+        _check(lib().hp3d_server_image_read(self.h, handle, level, ctypes.byref(W),
+                                            ctypes.byref(H), _p(out, _f32p), out.size))
+        return out
+
     def submit(self, ref, srcs, K_ref_inv, A, b, depth_min, depth_max, iterations, seed=42,
-               top_k=3, with_normal=False, with_cost=True, fail_at_step=0):
-        """Queue one job; returns its id.  Output arrays are filled on completion."""
-        ref = np.ascontiguousarray(ref, dtype=np.uint8)
-        srcs = [np.ascontiguousarray(s, dtype=np.uint8) for s in srcs]
-        H, W = ref.shape
+               top_k=3, with_normal=False, with_cost=True, fail_at_step=0, level=0, shape=None):
+        """Queue one job; returns its id.  Output arrays are filled on completion.
+        ref and each source: an (H, W) uint8 image, or the handle of a resident
+        image (then shape = (H, W) at `level`)."""
+        keep = []
         d = JobDesc()
-        d.width, d.height, d.num_src = W, H, len(srcs)
-        d.ref = _p(ref, _u8p)
-        for i, s in enumerate(srcs):
-            d.src[i] = _p(s, _u8p)
+        d.num_src = len(srcs)
+        d.level = int(level)
+        H = W = None
+        for k, im in enumerate([ref] + list(srcs)):
+            if isinstance(im, (int, np.integer)):
+                if k == 0:
+                    d.ref_image = int(im)
+                else:
+                    d.src_image[k - 1] = int(im)
+                continue
+            a = np.ascontiguousarray(im, dtype=np.uint8)
+            keep.append(a)
+            H, W = a.shape
+            if k == 0:
+                d.ref = _p(a, _u8p)
+            else:
+                d.src[k - 1] = _p(a, _u8p)
+        if shape is not None:
+            H, W = shape
+        if H is None:
+            raise ValueError("shape is needed when every image is resident")
+        d.width, d.height = W, H
         for i, val in enumerate(np.asarray(K_ref_inv, dtype=np.float64).reshape(9)):
             d.K_ref_inv[i] = val
         for s in range(len(srcs)):

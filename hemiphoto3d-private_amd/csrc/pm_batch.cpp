@@ -8,7 +8,9 @@
 // view that still has one, and each view writes its outputs in its last
 // step.  The step launches are captured once in a hipGraph; a later batch
 // with the same schedule shape replays the graph.
+#include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <fstream>
@@ -61,13 +63,34 @@ void bind_job(DevJob& j, const hp3d_view_desc* d, const Image* ref, const Image*
     j.seed = d->seed;
 }
 
-int tune_key(int radius, int stride, int store) { return (radius * 4 + stride) * 2 + store; }
+// The autotuner's choice is kept per (window radius, stride, storage, size
+// bucket, mean source count): the size bucket is log2 of the batch's pixels
+// in units of 2^18 (a quarter megapixel), 0..15, and the mean source count
+// is rounded to 1..8 -- the quantities that move the best tile shape (tail
+// length per step, work per tile).
+int tune_key(int radius, int stride, int store, double pixels, double mean_src)
+{
+    int bucket = 0;
+    while (bucket < 15 && pixels >= (double)(1ull << (19 + bucket))) ++bucket;
+    const int S = std::max(1, std::min(8, (int)std::lround(mean_src)));
+    return ((((radius * 4 + stride) * 2 + store) * 16 + bucket) * 9) + S;
+}
 
-int chosen_variant(const hp3d_pm_ctx* ctx, int radius, int stride, int store, int requested)
+int batch_tune_key(const hp3d_view_desc* descs, const std::vector<const Image*>& refs, int n)
+{
+    double px = 0.0, src = 0.0;
+    for (int j = 0; j < n; ++j) {
+        px += (double)refs[j]->W * refs[j]->H;
+        src += descs[j].num_src;
+    }
+    return tune_key(descs[0].radius, descs[0].stride, refs[0]->store, px, src / n);
+}
+
+int chosen_variant(const hp3d_pm_ctx* ctx, int radius, int stride, int key, int requested)
 {
     if (!step_variant_tunable(radius, stride)) return 0;
     if (requested >= 0) return requested;
-    auto it = ctx->tuned.find(tune_key(radius, stride, store));
+    auto it = ctx->tuned.find(key);
     return it == ctx->tuned.end() ? kDefaultStepVariant : it->second;
 }
 
@@ -253,7 +276,8 @@ int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
         if (!outs[j].depth) return ctx->fail(HP3D_ERR_INVALID_ARG, "batch view %d: no depth output", j);
     (void)hipSetDevice(ctx->device);
     const int radius = descs[0].radius, stride = descs[0].stride, store = refs[0]->store;
-    const int variant = chosen_variant(ctx, radius, stride, store, opts ? opts->variant : -1);
+    const int variant = chosen_variant(ctx, radius, stride, batch_tune_key(descs, refs, n),
+                                       opts ? opts->variant : -1);
     if (variant >= kNumStepVariants)
         return ctx->fail(HP3D_ERR_INVALID_ARG, "variant %d outside 0..%d", variant, kNumStepVariants - 1);
     const bool use_graph = opts ? opts->use_graph != 0 : true;
@@ -424,7 +448,7 @@ int hp3d_pm_autotune(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n, int* 
     (void)hipEventDestroy(ev0);
     (void)hipEventDestroy(ev1);
     if (rc) return rc;
-    ctx->tuned[tune_key(radius, stride, store)] = best;
+    ctx->tuned[batch_tune_key(descs, refs, n)] = best;
     if (best_variant) *best_variant = best;
     return HP3D_OK;
 }

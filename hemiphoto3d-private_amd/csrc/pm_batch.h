@@ -18,7 +18,8 @@ void bind_job(DevJob& j, const hp3d_view_desc* d, const Image* ref, const Image*
               char* base, int outputs_mask);
 // Step `step` (0 = start) of a view with `iterations` iterations.
 StepItem make_item(int job, int step, int iterations, uint64_t seed, int W, int H, int variant);
-int tune_key(int radius, int stride, int store);
-int chosen_variant(const hp3d_pm_ctx* ctx, int radius, int stride, int store, int requested);
+int tune_key(int radius, int stride, int store, double pixels, double mean_src);
+int batch_tune_key(const hp3d_view_desc* descs, const std::vector<const Image*>& refs, int n);
+int chosen_variant(const hp3d_pm_ctx* ctx, int radius, int stride, int key, int requested);
 
 }  // namespace hp3d

@@ -142,6 +142,40 @@ __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, con
     }
 }
 
+// One pyramid level from the previous one: element (x, y) is the mean of
+// the 2 x 2 block at (2x, 2y) of the previous level's plain plane (stored
+// DN - 128 values), written as the level's plain and row-pair planes.  The
+// sums are exact in FP32 (values carry 2 more fraction bits per level); FP16
+// storage is exact for the first two levels.
+template <int STORE>
+__global__ __launch_bounds__(256) void img_down_kernel(const void* prev, int prev_pitch, int W,
+                                                       int H, void* plain, void* coef, int pitch)
+{
+    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    if (x >= W || y >= H) return;
+    auto val = [&](int xx, int yy) -> float {
+        const int q = 2 * yy * prev_pitch + 2 * xx;
+        const float a = load_plain<STORE>(prev, q), b = load_plain<STORE>(prev, q + 1);
+        const float c = load_plain<STORE>(prev, q + prev_pitch);
+        const float d = load_plain<STORE>(prev, q + prev_pitch + 1);
+        return ((a + b) + (c + d)) * 0.25f;
+    };
+    const float a = val(x, y);
+    const float n = y < H - 1 ? val(x, y + 1) : 0.0f;
+    const size_t q = (size_t)y * pitch + x;
+    if (STORE == kStoreF32) {
+        static_cast<float*>(plain)[q] = a;
+        static_cast<float2*>(coef)[q] = make_float2(a, n);
+    } else {
+        static_cast<_Float16*>(plain)[q] = (_Float16)a;
+        half2_t h;
+        h.x = (_Float16)a;
+        h.y = (_Float16)n;
+        static_cast<half2_t*>(coef)[q] = h;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Host-side launchers
 // ---------------------------------------------------------------------------
@@ -177,6 +211,18 @@ hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_
     else
         hipLaunchKernelGGL(img_prep_kernel<kStoreF16>, grid_2d(W, H), dim3(256), 0, st, in_u8,
                            in_f32, in_pitch, W, H, plain, coef, pitch);
+    return hipGetLastError();
+}
+
+hipError_t launch_img_down(const void* prev, int prev_pitch, int W, int H, int store, void* plain,
+                           void* coef, int pitch, hipStream_t st)
+{
+    if (store == kStoreF32)
+        hipLaunchKernelGGL(img_down_kernel<kStoreF32>, grid_2d(W, H), dim3(256), 0, st, prev,
+                           prev_pitch, W, H, plain, coef, pitch);
+    else
+        hipLaunchKernelGGL(img_down_kernel<kStoreF16>, grid_2d(W, H), dim3(256), 0, st, prev,
+                           prev_pitch, W, H, plain, coef, pitch);
     return hipGetLastError();
 }
 

@@ -19,6 +19,10 @@ hipError_t launch_import(const DevView& v, const float* depth, const float* norm
 // coef: the row-pair gather plane; both planes use `pitch` elements per row.
 hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_pitch, int W,
                            int H, int store, void* plain, void* coef, int pitch, hipStream_t st);
+// Pyramid level (W x H) from the previous level's plain plane (2W x 2H or
+// larger, prev_pitch elements per row): 2 x 2 box means, both planes.
+hipError_t launch_img_down(const void* prev, int prev_pitch, int W, int H, int store, void* plain,
+                           void* coef, int pitch, hipStream_t st);
 // SfM seeds (device array of n) painted over the hypothesis planes; owner is
 // W*H ints of scratch.
 hipError_t launch_seed(const DevView& v, const hp3d_seed_point* seeds, int n, int radius,

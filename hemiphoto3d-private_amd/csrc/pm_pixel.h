@@ -521,11 +521,14 @@ __device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, 
 }
 
 // Per-wave LDS scratch of a half-pass: the wave's list of neighbour
-// candidates that must be evaluated (owner lane | candidate << 6), and their
-// costs, at most 4 per lane; and the depth-bucket counts of the first
-// iteration's ordering.
+// candidates that must be evaluated (owner lane | candidate << 6), their
+// hypotheses (indexed the same way: each is loaded once, by its owner, and
+// read from here by the lane that evaluates it and by the owner when it
+// accepts it) and their costs, at most 4 per lane; and the depth-bucket
+// counts of the first iteration's ordering.
 struct WaveXchg {
     unsigned char item[256];
+    float4 hyp[256];
     float cost[256];
 #if HP3D_BUCKET
     int hist[64];
@@ -617,6 +620,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
             for (int j = 0; j < k; ++j) n = n && !(ok[j] && same(h[j], h[k]));
             need |= (unsigned)n << k;
             hd[k] = h[k].x;
+            if (n) xw.hyp[lane | (k << 6)] = h[k];
         }
     }
     const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
@@ -702,7 +706,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
             const int tx_o = t_o % TX, ty_o = t_o / TX;
             const int i_o = bx * TX + tx_o, y_o = by * TY + ty_o;
             const int o_o = (y_o + parity) & 1, x_o = 2 * i_o + o_o;
-            const float4 h = IO::ld4(oth, nb(k, y_o * Wc + i_o, o_o));
+            const float4 h = xw.hyp[it];
             double rx, ry, rz;
             ray_fp64(v.kinv_d, x_o, y_o, rx, ry, rz);
             xw.cost[g] = cand_cost<NT, ST, STORE>(v, tile + ty_o * G::LP + (x_o - 2 * bx * TX),
@@ -716,7 +720,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
     for (int k = 0; k < 4; ++k) {
         if (!((need >> k) & 1u)) continue;
         const float c = xw.cost[slot[k]];
-        if (c < bc) { bc = c; best = IO::ld4(oth, nb(k, p, o)); changed = true; }
+        if (c < bc) { bc = c; best = xw.hyp[lane | (k << 6)]; changed = true; }
     }
 
     double rx, ry, rz;

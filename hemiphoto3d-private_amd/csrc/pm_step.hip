@@ -29,6 +29,7 @@
 namespace hp3d {
 
 constexpr int kStepChunk = 16;   // tiles per chunk of the XCD interleave
+constexpr int kPersistMaxGrid = 4096;
 
 template <int NT, int ST, int STORE, int TX, int TY, int WPE>
 __global__ __launch_bounds__(TX * TY) __attribute__((amdgpu_waves_per_eu(WPE, 8)))
@@ -239,20 +240,12 @@ static hipError_t launch_persist_t(const DevJob* jobs, const StepItem* items, in
                                    int total_units, int* done, int n_counters, int chunk,
                                    unsigned long long wait_ticks, hipStream_t st)
 {
-    // as many workgroups as fit the GPU at once (more would only start after
-    // the work is gone); computed once per kernel instance
-    static int grid = 0;
-    if (grid == 0) {
-        int dev = 0, cus = 0, per_cu = 0;
-        hipError_t e = hipGetDevice(&dev);
-        if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        if (e == hipSuccess)
-            e = hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                &per_cu, reinterpret_cast<const void*>(&pm_persist_kernel<NT, ST, STORE, TX, TY, WPE>),
-                TX * TY, 0);
-        if (e != hipSuccess) return e;
-        grid = std::max(1, cus * std::max(per_cu, 1));
-    }
+    // More workgroups than the GPU holds at once (256 CUs x at most 8 of
+    // these per CU): the dispatcher keeps every CU full, the work counter
+    // spreads the units, and workgroups started after the work is gone exit
+    // at once.  No co-residency is assumed (see the progress argument above).
+    const int chunks = (total_units + chunk - 1) / chunk;
+    const int grid = std::max(1, std::min(chunks, kPersistMaxGrid));
     hipLaunchKernelGGL((pm_persist_kernel<NT, ST, STORE, TX, TY, WPE>), dim3(grid), dim3(TX * TY),
                        0, st, jobs, items, n_items, total_units, done, n_counters, chunk,
                        wait_ticks);

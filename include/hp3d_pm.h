@@ -252,7 +252,9 @@ int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,
 
 /* Autotuner: times every step-kernel variant (tile shape and workgroup size)
  * on the given views for one start step and two half-passes, on scratch
- * state, and records the fastest for their (radius, stride, storage).
+ * state, and records the fastest for their (radius, stride, storage, total
+ * pixels in power-of-two buckets from 0.5 Mpixel, mean source count); a later
+ * batch with the same key runs that variant.
  * times_ms (nullable) receives HP3D_PM_NUM_VARIANTS entries, -1 where a
  * variant does not apply. */
 #define HP3D_PM_NUM_VARIANTS 6
@@ -269,9 +271,11 @@ int hp3d_pm_tune_load(hp3d_pm_ctx* ctx, const char* path);
  * reference view each, with its own images) are submitted at any time; a
  * scheduler thread admits them into free slots when their device footprint
  * (images + state + outputs) fits the server's HBM budget, and launches one
- * step at a time over every admitted job, each at its own pass.  Jobs join
- * and leave between steps; images, hypothesis state and outputs stay on the
- * device until the job completes.  A job's result is bit-identical to
+ * step at a time over every admitted job, each at its own pass, with up to
+ * two steps queued on the device so the scheduler prepares the next step
+ * while one runs.  Jobs join and leave between steps; images, hypothesis
+ * state and outputs stay on the device until the job completes, and a
+ * completion thread downloads finished jobs' outputs off the scheduler.  A job's result is bit-identical to
  * hp3d_pm_run_view on the same view, whatever else runs beside it.
  * There is no counterpart in the reference, whose MVS runs only inside the
  * blocking runPatchMatchMVS call (patchmatch_mvs.cu:1028).
@@ -288,8 +292,10 @@ typedef struct {
 } hp3d_server_opts;
 
 typedef struct {
-    int width, height, num_src;
-    const uint8_t* ref;                      /* host 8-bit images, copied at submit */
+    int width, height, num_src;              /* width, height: of the job's images (at
+                                                `level` for resident images) */
+    const uint8_t* ref;                      /* host 8-bit images, copied at submit; ignored
+                                                where a resident image is named below */
     const uint8_t* src[HP3D_PM_MAX_SRC];
     double K_ref_inv[9];
     double A[HP3D_PM_MAX_SRC][9];
@@ -301,6 +307,12 @@ typedef struct {
     float* out_normal;                       /* host, H*W*3, nullable */
     float* out_cost;                         /* host, H*W, nullable */
     int fail_at_step;                        /* test hook: > 0 fails the job at that step */
+    uint64_t ref_image;                      /* resident image (hp3d_server_image_add) used as
+                                                the reference instead of `ref`; 0 = none */
+    uint64_t src_image[HP3D_PM_MAX_SRC];     /* resident images used as sources; 0 = none */
+    int level;                               /* pyramid level of the resident images (0 = full
+                                                size); K_ref_inv, A, b are the caller's for
+                                                that level's pixel grid */
 } hp3d_job_desc;
 
 typedef enum {
@@ -326,6 +338,10 @@ typedef struct {
     double mpix_iters;          /* W*H*iterations of completed jobs / 1e6 */
     size_t arena_bytes, arena_peak;
     int max_active;             /* most jobs in one step */
+    long long image_uploads;    /* images uploaded: per-job host images and resident images */
+    long long image_reuses;     /* job images served by a resident image (no upload) */
+    long long resident_images;  /* resident images currently held */
+    double wait_ms;             /* scheduler time spent waiting for a step to finish */
 } hp3d_server_stats;
 
 int hp3d_server_create(const hp3d_server_opts* opts, hp3d_pm_server** out);
@@ -343,6 +359,20 @@ int hp3d_server_cancel(hp3d_pm_server* srv, uint64_t id);
  * counterpart in the reference, which has no serving loop.) */
 int hp3d_server_forget(hp3d_pm_server* srv, uint64_t id);
 int hp3d_server_get_stats(hp3d_pm_server* srv, hp3d_server_stats* st);
+/* Resident images: one 8-bit image uploaded once, with `levels` (1..4)
+ * pyramid levels -- level l is the 2x2 box average of level l - 1, of size
+ * (W >> l) x (H >> l), in the server's storage -- kept in the server's HBM
+ * budget until removed, for any number of jobs to use by handle (the
+ * reference keeps every registered image resident for its whole run,
+ * patchmatch_mvs.cu:1042-1055).  Blocks until the upload is done;
+ * HP3D_ERR_OOM_BUDGET if the planes do not fit the budget now.  A removed
+ * image's memory is freed once no queued or running job uses it. */
+int hp3d_server_image_add(hp3d_pm_server* srv, const uint8_t* gray, int width, int height,
+                          size_t pitch, int levels, uint64_t* image);
+int hp3d_server_image_remove(hp3d_pm_server* srv, uint64_t image);
+/* A resident image's level as DN values (gray * 255, H_l x W_l floats). */
+int hp3d_server_image_read(hp3d_pm_server* srv, uint64_t image, int level, int* width,
+                           int* height, float* out, size_t capacity);
 const char* hp3d_server_last_error(hp3d_pm_server* srv);
 int hp3d_server_destroy(hp3d_pm_server* srv);
 

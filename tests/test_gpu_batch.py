@@ -97,3 +97,47 @@ def test_batch_rejects_mixed_windows(pkg, ctx, mixed_jobs, singles):
     with pytest.raises(pkg.HP3DError) as e:
         ctx.batch_run([descs[0], bad], [mixed_jobs[0].shape, mixed_jobs[1].shape])
     assert e.value.code == pkg.ERR_INVALID_ARG
+
+
+# ---------------------------------------------------------------------------
+# Persistent form: one kernel for the whole batch, tiles released by per-tile
+# counters (a half-pass of a tile waits for that tile and its 4 neighbours
+# to finish the previous pass).  Same pixels, same arithmetic: every view
+# must equal its per-view solve bit for bit, for every tile shape and chunk
+# size, from a graph replay too (the counters are reset on every replay).
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("variant,chunk", [(0, 1), (1, 8), (2, 3), (3, 16), (4, 8), (5, 64)])
+def test_persistent_batch_matches_per_view(pkg, ctx, mixed_jobs, singles, variant, chunk):
+    descs, ref_out = singles
+    shapes = [j.shape for j in mixed_jobs]
+    print(f"persistent variant {variant} chunk {chunk}", flush=True)
+    outs = ctx.batch_run(descs, shapes, variant=variant, persistent=True, chunk=chunk)
+    for k, (o, r) in enumerate(zip(outs, ref_out)):
+        for key in ("depth", "normal", "cost"):
+            assert np.array_equal(o[key], r[key]), (variant, chunk, k, key)
+
+
+def test_persistent_batch_graph_replays_and_direct(pkg, ctx, mixed_jobs, singles):
+    descs, ref_out = singles
+    shapes = [j.shape for j in mixed_jobs]
+    for use_graph in (True, True, True, False):
+        outs = ctx.batch_run(descs, shapes, variant=1, use_graph=use_graph, persistent=True)
+        for o, r in zip(outs, ref_out):
+            assert np.array_equal(o["depth"], r["depth"]) and np.array_equal(o["cost"], r["cost"])
+
+
+def test_persistent_batch_f16_and_other_window(pkg, ctx, scenes, mixed_jobs):
+    """FP16 storage, and a 5x5 window at stride 2 (the 16x16-tile kernel)."""
+    descs16 = descs_for(pkg, ctx, mixed_jobs, store=1)
+    outs = ctx.batch_run(descs16, [j.shape for j in mixed_jobs], persistent=True)
+    for d, o, j in zip(descs16, outs, mixed_jobs):
+        r = ctx.run_view(d, j.shape)
+        assert np.array_equal(o["depth"], r["depth"]) and np.array_equal(o["cost"], r["cost"])
+    jobs = [scenes.sphere_job(W, H, S, scene_seed=90 + k, radius=2, stride=2, iterations=it,
+                              seed=5 + k, name=f"r2s2_{k}")
+            for k, (W, H, S, it) in enumerate([(130, 90, 3, 2), (77, 140, 6, 3), (200, 64, 2, 1)])]
+    descs = descs_for(pkg, ctx, jobs)
+    outs = ctx.batch_run(descs, [j.shape for j in jobs], persistent=True, chunk=2)
+    for d, o, j in zip(descs, outs, jobs):
+        r = ctx.run_view(d, j.shape)
+        assert np.array_equal(o["depth"], r["depth"]) and np.array_equal(o["normal"], r["normal"])

@@ -186,3 +186,94 @@ def test_cancel_queued_job_releases_nothing(pkg, ctx, serve_jobs, expected):
         s = srv.stats()
         print("arena", fp, budget, s["arena_peak"])
         assert s["arena_peak"] <= budget
+
+
+# ---------------------------------------------------------------------------
+# Resident images and pyramid levels
+# ---------------------------------------------------------------------------
+def test_resident_images_match_host_images(pkg, serve_jobs, expected):
+    """Jobs that name resident images give the same bits as jobs that send
+    their images; each resident image is uploaded once however many jobs
+    use it, and the stats count the reuses."""
+    job = serve_jobs[2]
+    with pkg.Server(256 * MB, max_jobs=4, stride=2) as srv:
+        ref = srv.image_add(job.ref)
+        srcs = [srv.image_add(s) for s in job.srcs]
+        ids = [srv.submit(ref, srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax, job.iterations,
+                          seed=job.seed, shape=job.shape) for _ in range(3)]
+        mixed = srv.submit(job.ref, srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                           job.iterations, seed=job.seed)      # host reference, resident sources
+        for jid in ids + [mixed]:
+            assert srv.wait(jid, 60)["state"] == pkg.JOB_DONE
+            _check(srv.result(jid), expected[2])
+        s = srv.stats()
+        print("resident stats", s)
+        assert s["image_uploads"] == 1 + len(srcs) + 1      # resident images + one host image
+        assert s["image_reuses"] == 3 * (1 + len(srcs)) + len(srcs)
+        assert s["resident_images"] == 1 + len(srcs)
+
+
+def test_remove_resident_image_while_in_use(pkg, serve_jobs, expected):
+    job = serve_jobs[0]
+    with pkg.Server(256 * MB, max_jobs=2, stride=2) as srv:
+        hs = [srv.image_add(im) for im in [job.ref] + job.srcs]
+        jid = srv.submit(hs[0], hs[1:], job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                         job.iterations, seed=job.seed, shape=job.shape)
+        for h in hs:
+            srv.image_remove(h)                    # still used by the job: freed afterwards
+        assert srv.wait(jid, 60)["state"] == pkg.JOB_DONE
+        _check(srv.result(jid), expected[0])
+        assert srv.stats()["resident_images"] == 0
+        with pytest.raises(pkg.HP3DError) as e:
+            srv.submit(hs[0], hs[1:], job.Kinv, job.A, job.b, job.dmin, job.dmax, 1,
+                       shape=job.shape)
+        assert e.value.code == pkg.ERR_NOT_FOUND
+
+
+@pytest.mark.parametrize("store", [0, 1], ids=["f32", "f16"])
+def test_pyramid_levels_are_box_means(pkg, scenes, store):
+    """Level l is the 2x2 mean of level l - 1 (sizes halved, odd edges
+    dropped): exact in FP32 at every level, in FP16 for the first two."""
+    img = np.random.default_rng(4).integers(0, 256, (203, 310), dtype=np.uint8)
+    with pkg.Server(64 * MB, max_jobs=1, stride=2, store=store) as srv:
+        h = srv.image_add(img, levels=4)
+        prev = img.astype(np.float64)
+        assert np.array_equal(srv.image_read(h, 0), prev)
+        for lv in range(1, 4):
+            Hl, Wl = prev.shape[0] // 2, prev.shape[1] // 2
+            want = prev[:2 * Hl, :2 * Wl].reshape(Hl, 2, Wl, 2).mean(axis=(1, 3))
+            got = srv.image_read(h, lv).astype(np.float64)
+            assert got.shape == want.shape
+            if store == 0 or lv <= 2:
+                assert np.array_equal(got, want), lv
+            else:
+                assert np.abs(got - want).max() <= 0.0625, lv
+            prev = got
+
+
+def test_job_on_a_pyramid_level_matches_the_small_image(pkg, orc, ctx, scenes):
+    """A 2x pixel-replicated image has the original as its first pyramid
+    level, exactly: a job on level 1 of replicated images must give the
+    bits of the per-view solve on the originals, and that solve agrees with
+    the oracle's run."""
+    job = scenes.sphere_job(160, 120, 3, scene_seed=71, radius=5, stride=2, iterations=2,
+                            seed=9, name="level1")
+    up = [np.repeat(np.repeat(im, 2, axis=0), 2, axis=1) for im in [job.ref] + job.srcs]
+    ref = ctx.upload(job.ref, pkg.STORE_F16)
+    srcs = [ctx.upload(s, pkg.STORE_F16) for s in job.srcs]
+    exp = ctx.run_view(pkg.make_desc(ref, srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                                     radius=5, stride=2, iterations=job.iterations, seed=job.seed),
+                       job.shape)
+    for h in [ref] + srcs:
+        ctx.release(h)
+    with pkg.Server(64 * MB, max_jobs=2, stride=2) as srv:
+        hs = [srv.image_add(im, levels=2) for im in up]
+        jid = srv.submit(hs[0], hs[1:], job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                         job.iterations, seed=job.seed, level=1, shape=job.shape)
+        assert srv.wait(jid, 60)["state"] == pkg.JOB_DONE
+        _check(srv.result(jid), exp)
+    v = orc.View(job.ref, job.srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax, radius=5,
+                 stride=2, top_k=3)
+    o64, _, _ = v.patchmatch_fp32(job.iterations, job.seed, cost_mode=orc.COST_FP64)
+    agree = (np.abs(exp["depth"].astype(np.float64) - o64) <= 0.01 * np.abs(o64)).mean()
+    assert agree >= 0.99, agree
