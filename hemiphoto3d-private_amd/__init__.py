@@ -187,6 +187,8 @@ SIGNATURES = {
                                         ctypes.POINTER(ViewOut)]),
     "hp3d_pm_run_views": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc), ctypes.c_int,
                                          ctypes.POINTER(ViewOut)]),
+    "hp3d_pm_run_view_noalloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.POINTER(ViewDesc),
+                                                ctypes.POINTER(ViewOut)]),
     "hp3d_pm_host_alloc": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_size_t,
                                           ctypes.POINTER(ctypes.c_void_p)]),
     "hp3d_pm_host_free": (ctypes.c_int, [ctypes.c_void_p, ctypes.c_void_p]),

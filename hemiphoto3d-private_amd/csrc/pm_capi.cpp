@@ -261,6 +261,7 @@ int hp3d_pm_create(const hp3d_pm_opts* opts, hp3d_pm_ctx** out)
         return HP3D_ERR_DEVICE;
     }
     ctx->hbm_budget = o.hbm_budget ? o.hbm_budget : (size_t)(0.9 * (double)free_b);
+    noalloc_setup(o.device);
     *out = ctx;
     return HP3D_OK;
 }

@@ -152,6 +152,9 @@ int solve_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
 // copy stream while view i + 1 computes.  srcs[i] holds view i's sources.
 int solve_views(hp3d_pm_ctx* ctx, int n, const hp3d_view_desc* d, const Image* const* refs,
                 const Image* const* const* srcs, const hp3d_view_out* out);
+// Resolve this device's copy of the no-allocation scratch array (once per device).
+void noalloc_setup(int device);
+
 int init_only(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,
               const Image* const* srcs, float* depth, float* normal);
 int cost_only(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image* ref,

@@ -159,6 +159,20 @@ int hp3d_pm_run_view(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, const hp3d_vi
 int hp3d_pm_run_views(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc, int n,
                       const hp3d_view_out* out);
 
+/* The solve of hp3d_pm_run_view for callers that cannot allocate device
+ * memory (code under hipGraph capture, serving loops that forbid hipMalloc).
+ * No counterpart in the reference.  The view's hypothesis state lives in a
+ * fixed device array of HP3D_PM_NOALLOC_SCRATCH_BYTES that the library holds
+ * on every GPU it runs on until the process ends; the call allocates nothing.
+ * out must be device buffers (on_device = 1); caller hypotheses and SfM seeds
+ * are refused (they need staging memory), as is a view whose state (21 bytes
+ * per pixel) exceeds the array: about 3.2 Mpixel.  Results equal
+ * hp3d_pm_run_view's.  Calls that overlap on one GPU share the array and
+ * race: one call at a time per GPU. */
+#define HP3D_PM_NOALLOC_SCRATCH_BYTES (64u << 20)
+int hp3d_pm_run_view_noalloc(hp3d_pm_ctx* ctx, const hp3d_view_desc* desc,
+                             const hp3d_view_out* out);
+
 /* Page-locked host memory owned by the context, for output buffers that
  * downloads write at full link speed (the reference returns cv::Mat in
  * pageable memory, patchmatch_mvs.cu:1243-1252).  Freed by hp3d_pm_host_free
