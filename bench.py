@@ -74,6 +74,10 @@ def parse():
                     help="batched/serving: step-kernel variant (-1: autotuned / default)")
     ap.add_argument("--form-store", choices=["f32", "f16"], default="f16",
                     help="batched/serving image storage")
+    ap.add_argument("--no-forms", action="store_true",
+                    help="plain line only: skip the batched and serving sub-objects")
+    ap.add_argument("--serve-seconds", type=float, default=15.0,
+                    help="arrival window of each serving sub-object of the default line")
     return ap.parse_args()
 
 
@@ -184,6 +188,16 @@ def main():
     cpu = None
     if not a.no_cpu:
         cpu = cpu_baseline(entry, job, a.cpu_threads, k)
+    forms = {}
+    if not a.no_forms:
+        ctx.close()
+        print("bench: batched workload", file=sys.stderr, flush=True)
+        forms["batched"] = measure_batched(a, entry, pkg, scenes, views=48, stride=1, reps=1,
+                                           with_fp32=False, with_cpu=False)
+        for stride in (2, 1):
+            print(f"bench: serving workload, stride {stride}", file=sys.stderr, flush=True)
+            forms[f"serving_stride{stride}"] = measure_serving(
+                a, entry, pkg, scenes, rate=20.0, duration=a.serve_seconds, stride=stride)
 
     lat = np.array(lat)
     out = {
@@ -213,8 +227,10 @@ def main():
         "device_outputs_value": round(device_value, 2),
         "serial_host_value": round(serial_value, 2),
     }
+    out.update(forms)
     print(json.dumps(out))
-    ctx.close()
+    if a.no_forms:
+        ctx.close()
 
 
 def cpu_baseline(entry, job, threads, k):
@@ -359,13 +375,15 @@ def form_precision(a, entry, pkg, scenes):
     ctx.close()
 
 
-def form_batched(a, entry, pkg, scenes):
-    """48 views, resolutions uniform over {640x480 ... 4000x3000}, 3-8 sources,
-    11x11 window, 8 iterations, FP16 storage, one hipGraph per batch, tile
-    shape chosen by the autotuner; against the plain FP32 kernel run view by
-    view over the same views."""
-    stride = a.stride or 1
-    specs = scenes.batch_specs(a.views, iterations=8, stride=stride)
+def measure_batched(a, entry, pkg, scenes, views=48, stride=1, reps=2, with_fp32=True,
+                    with_cpu=True):
+    """The batched workload: `views` views, resolutions uniform over {640x480 ...
+    4000x3000}, 3-8 sources, 11x11 window, 8 iterations, FP16 storage, the
+    whole batch in one launch captured in a hipGraph -- the persistent kernel
+    and the graph of step launches both timed, the faster reported -- tile
+    shape chosen by the autotuner; against the plain kernel run view by view
+    over the same views.  Outputs stay in device memory."""
+    specs = scenes.batch_specs(views, iterations=8, stride=stride)
     t0 = time.perf_counter()
     jobs = [scenes.job_from_spec(s, threads=a.cpu_threads) for s in specs]
     render_s = time.perf_counter() - t0
@@ -389,7 +407,6 @@ def form_batched(a, entry, pkg, scenes):
         best, times = a.variant, np.full(pkg.NUM_VARIANTS, -1.0)
     else:
         best, times = ctx.autotune(d16)
-    reps = 2
 
     def timed_batch(**kw):
         ctx.batch_run_device(d16, outs, variant=best, **kw)     # warm-up (captures the graph)
@@ -401,7 +418,8 @@ def form_batched(a, entry, pkg, scenes):
     persist_s = timed_batch(persistent=True, chunk=a.chunk)
     steps_s = timed_batch(persistent=False)
     batched_s = min(persist_s, steps_s)
-    # plain kernel one view after the other: FP32 storage, and the batch's storage
+
+    # plain kernel one view after the other: the batch's storage (and FP32)
     def serial(descs):
         ctx.run_view_device(descs[0], *outs[0])
         t0 = time.perf_counter()
@@ -409,10 +427,11 @@ def form_batched(a, entry, pkg, scenes):
             ctx.run_view_device(d, *o)
         return time.perf_counter() - t0
     plain_same_s = serial(d16)
-    d32 = descs_for(pkg.STORE_F32) if fstore != pkg.STORE_F32 else d16
-    plain_s = serial(d32)
+    plain_s = None
+    if with_fp32 and fstore != pkg.STORE_F32:
+        plain_s = serial(descs_for(pkg.STORE_F32))
     cpu = None
-    if not a.no_cpu:
+    if with_cpu and not a.no_cpu:
         rate = _cpu_tap_rate(entry, a.cpu_threads)
         cpu = {"value": round(mpix / (taps / rate), 3), "unit": "Mpixel-iters/s",
                "cores": a.cpu_threads, "kind": "port",
@@ -421,51 +440,67 @@ def form_batched(a, entry, pkg, scenes):
     ci = [ctypes.c_int() for _ in range(3)]
     pkg.lib().hp3d_pm_variant_info(best, *[ctypes.byref(c) for c in ci])
     value = mpix / batched_s
-    out = {"form": "batched", "metric": "PatchMatch Mpixel-iters/s", "value": round(value, 2),
-           "unit": "Mpixel-iters/s", "dtype": f"{a.form_store} storage / fp32 accumulation",
-           "data": "synthetic", "config": {
-               "workload": f"batched form: {a.views} views, resolutions over "
-                           "{640x480,1280x960,1600x1200,2400x1800,4000x3000}, 3-8 sources, "
-                           f"11x11 stride {stride}, 8 iterations, one graph launch",
-               "views": a.views, "mpixel_iters": round(mpix, 2), "taps": taps,
-               "render_s": round(render_s, 1)},
+    out = {"value": round(value, 2), "unit": "Mpixel-iters/s",
+           "dtype": f"{a.form_store} storage / fp32 accumulation",
+           "config": {"workload": f"batched form: {views} views, resolutions over "
+                                  "{640x480,1280x960,1600x1200,2400x1800,4000x3000}, 3-8 sources, "
+                                  f"11x11 stride {stride}, 8 iterations, one launch in a hipGraph",
+                      "views": views, "mpixel_iters": round(mpix, 2), "taps": taps,
+                      "render_s": round(render_s, 1)},
            "ms_per_batch": round(batched_s * 1e3, 2),
-           "persistent_value": round(mpix / persist_s, 2),
-           "persistent_chunk": a.chunk,
-           "step_graph_value": round(mpix / steps_s, 2),
            "launch_form": "persistent kernel" if persist_s <= steps_s else "graph of step launches",
-           "plain_serial_value": round(mpix / plain_s, 2),
-           "speedup_vs_plain_serial": round(plain_s / batched_s, 3),
+           "persistent_value": round(mpix / persist_s, 2), "persistent_chunk": a.chunk,
+           "step_graph_value": round(mpix / steps_s, 2),
            "plain_serial_same_storage_value": round(mpix / plain_same_s, 2),
+           "speedup_vs_plain_serial_same_storage": round(plain_same_s / batched_s, 3),
            "autotune": {"variant": best, "tile": [ci[0].value, ci[1].value],
-                        "waves_per_simd_hint": ci[2].value,
                         "times_ms": [round(float(t), 3) for t in times]},
            "tflops_algorithmic": round(taps * FLOP_PER_TAP / batched_s / 1e12, 3),
            "cpu_baseline": cpu}
+    if plain_s:
+        out["plain_serial_value"] = round(mpix / plain_s, 2)
+        out["speedup_vs_plain_serial"] = round(plain_s / batched_s, 3)
     if cpu:
         out["speedup_vs_cpu"] = round(value / cpu["value"], 1)
-    print(json.dumps(out))
     ctx.close()
+    return out
 
 
-def form_serving(a, entry, pkg, scenes):
-    """Poisson arrivals (rate, duration); each job 1 ref + 3-8 sources at a
-    resolution of the batched mix, 4-12 iterations, FP16 storage, continuous
-    batching with HBM-budgeted admission.  Latency = completion - submission."""
-    stride = a.stride or 2
+def form_batched(a, entry, pkg, scenes):
+    out = measure_batched(a, entry, pkg, scenes, views=a.views, stride=a.stride or 1)
+    out = dict(form="batched", metric="PatchMatch Mpixel-iters/s", data="synthetic", **out)
+    print(json.dumps(out))
+
+
+def measure_serving(a, entry, pkg, scenes, rate=20.0, duration=120.0, stride=2):
+    """The serving workload: Poisson arrivals (rate, duration); each job 1
+    reference + 3-8 sources from a pool of 16 views of the batched resolution
+    mix, 4-12 iterations, FP16 storage, continuous batching with HBM-budgeted
+    admission; the pool's images are resident on the device (added once,
+    named by handle).  Latency = completion - submission.  rho = offered
+    load / the plain kernel's rate run serially on the pool (same storage);
+    the device's busy fraction is the measured utilisation."""
     rng = np.random.default_rng(2024)
     pool_specs = scenes.batch_specs(16, batch_seed=77, iterations=8, stride=stride)
     pool = [scenes.job_from_spec(s, threads=a.cpu_threads) for s in pool_specs]
-    n_arr = rng.poisson(a.rate * a.duration)
-    arrivals = np.sort(rng.uniform(0, a.duration, n_arr))
+    n_arr = rng.poisson(rate * duration)
+    arrivals = np.sort(rng.uniform(0, duration, n_arr))
     picks = rng.integers(0, len(pool), n_arr)
     iters = rng.integers(4, 13, n_arr)
-    offered = sum(pool[p].shape[0] * pool[p].shape[1] * it for p, it in zip(picks, iters)) / 1e6 / a.duration
+    offered = sum(pool[p].shape[0] * pool[p].shape[1] * it
+                  for p, it in zip(picks, iters)) / 1e6 / duration
+    store = pkg.STORE_F16 if a.form_store == "f16" else pkg.STORE_F32
     srv = pkg.Server(hbm_budget=96 << 30, max_jobs=a.max_jobs, radius=5, stride=stride,
-                     store=pkg.STORE_F16 if a.form_store == "f16" else pkg.STORE_F32,
-                     variant=a.variant)
+                     store=store, variant=a.variant)
+    handles = [[srv.image_add(j.ref)] + [srv.image_add(x) for x in j.srcs] for j in pool]
+
+    def submit(p, it, **kw):
+        j = pool[p]
+        return srv.submit(handles[p][0], handles[p][1:], j.Kinv, j.A, j.b, j.dmin, j.dmax, it,
+                          top_k=j.top_k, shape=j.shape, **kw)
     for w in range(2):                                   # warm-up
-        srv.wait(srv.submit_job(pool[w], iterations=4), timeout=120)
+        srv.wait(submit(w, 4), timeout=120)
+    s0 = srv.stats()
     lat, done_mpix, t_first, t_last = [], 0.0, None, None
     pending = []
 
@@ -499,8 +534,7 @@ def form_serving(a, entry, pkg, scenes):
         if len(pending) > 400:                           # host memory guard when overloaded
             dropped += 1
             continue
-        pending.append((srv.submit_job(pool[picks[k]], iterations=int(iters[k]),
-                                       seed=1000 + k, with_cost=False),
+        pending.append((submit(int(picks[k]), int(iters[k]), seed=1000 + k, with_cost=False),
                         int(picks[k]), int(iters[k])))
         if k % 200 == 0:
             print(f"serving: {k}/{n_arr} submitted at t={now:.1f}s, in flight {len(pending)}",
@@ -516,12 +550,12 @@ def form_serving(a, entry, pkg, scenes):
     s = srv.stats()
     srv.close()
     wall = t_last - t_first
-    # the plain FP32 kernel, one job after the other, on the pool at 8 iterations
+    # the plain kernel, one job after the other, on the pool at 8 iterations
     ctx = pkg.Context()
     plain_mpix, plain_s = 0.0, 0.0
     for j in pool:
-        ref = ctx.upload(j.ref)
-        srcs = [ctx.upload(x) for x in j.srcs]
+        ref = ctx.upload(j.ref, store)
+        srcs = [ctx.upload(x, store) for x in j.srcs]
         d = pkg.make_desc(ref, srcs, j.Kinv, j.A, j.b, j.dmin, j.dmax, radius=5, stride=stride,
                           iterations=8, seed=j.seed)
         outs = _device_outputs(ctx, j.shape)
@@ -538,24 +572,36 @@ def form_serving(a, entry, pkg, scenes):
     ctx.close()
     lat = np.array(lat) * 1e3
     value = done_mpix / wall
-    out = {"form": "serving", "metric": "PatchMatch Mpixel-iters/s; p50/p99 job latency",
-           "value": round(value, 2), "unit": "Mpixel-iters/s", "max_jobs": a.max_jobs,
-           "variant": a.variant,
-           "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
-                          "p99": round(float(np.percentile(lat, 99)), 2),
-                          "mean": round(float(lat.mean()), 2)},
-           "dtype": f"{a.form_store} storage / fp32 accumulation", "data": "synthetic",
-           "config": {"workload": f"serving form: Poisson {a.rate}/s for {a.duration}s, 1 ref + "
-                                  "3-8 src from the batched resolution mix, 4-12 iterations, "
-                                  f"11x11 stride {stride}", "pool_jobs": len(pool)},
-           "arrivals": int(n_arr), "completed": len(lat), "failed": int(failed),
-           "dropped_overload": dropped,
-           "offered_mpix_iters_per_s": round(offered, 2),
-           "utilisation": round(s["busy_ms"] / 1e3 / s["wall_s"], 3),
-           "max_in_flight": int(max_queue), "max_active_per_step": s["max_active"],
-           "plain_serial_value": round(plain_mpix / plain_s, 2),
-           "speedup_vs_plain_serial": round(value / (plain_mpix / plain_s), 3),
-           "steps": s["steps"]}
+    plain_rate = plain_mpix / plain_s
+    steps = s["steps"] - s0["steps"]
+    return {"value": round(value, 2), "unit": "Mpixel-iters/s",
+            "latency_ms": {"p50": round(float(np.percentile(lat, 50)), 2),
+                           "p99": round(float(np.percentile(lat, 99)), 2),
+                           "mean": round(float(lat.mean()), 2)},
+            "dtype": f"{a.form_store} storage / fp32 accumulation",
+            "config": {"workload": f"serving form: Poisson {rate}/s for {duration}s, 1 ref + "
+                                   "3-8 src from the batched resolution mix, 4-12 iterations, "
+                                   f"11x11 stride {stride}, resident images",
+                       "pool_jobs": len(pool), "max_jobs": a.max_jobs},
+            "arrivals": int(n_arr), "completed": len(lat), "failed": int(failed),
+            "dropped_overload": dropped,
+            "offered_mpix_iters_per_s": round(offered, 2),
+            "rho": round(offered / plain_rate, 3),
+            "stable": bool(offered < plain_rate),
+            "utilisation": round((s["busy_ms"] - s0["busy_ms"]) / 1e3 / wall, 3),
+            "scheduler_wait_ms_per_step": round((s["wait_ms"] - s0["wait_ms"]) / max(steps, 1), 3),
+            "max_in_flight": int(max_queue), "max_active_per_step": s["max_active"],
+            "image_uploads": s["image_uploads"], "image_reuses": s["image_reuses"],
+            "plain_serial_same_storage_value": round(plain_rate, 2),
+            "speedup_vs_plain_serial": round(value / plain_rate, 3),
+            "steps": steps}
+
+
+def form_serving(a, entry, pkg, scenes):
+    out = measure_serving(a, entry, pkg, scenes, rate=a.rate, duration=a.duration,
+                          stride=a.stride or 2)
+    out = dict(form="serving", metric="PatchMatch Mpixel-iters/s; p50/p99 job latency",
+               data="synthetic", **out)
     print(json.dumps(out))
 
 
