@@ -63,7 +63,7 @@ def parse():
                     default="plain",
                     help="plain: the headline line; the others print their own JSON line")
     ap.add_argument("--views", type=int, default=48, help="batched: views per batch")
-    ap.add_argument("--chunk", type=int, default=8,
+    ap.add_argument("--chunk", type=int, default=16,
                     help="batched: units a persistent workgroup takes at a time")
     ap.add_argument("--rate", type=float, default=20.0, help="serving: arrivals per second")
     ap.add_argument("--duration", type=float, default=120.0, help="serving: arrival window (s)")

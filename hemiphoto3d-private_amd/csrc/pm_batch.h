@@ -7,7 +7,7 @@ namespace hp3d {
 
 // Persistent batch kernel: units a workgroup takes at a time, and the bound
 // on one dependency wait.
-constexpr int kDefaultPersistChunk = 8;
+constexpr int kDefaultPersistChunk = 16;
 constexpr int kPersistWaitSeconds = 10;
 
 // Bytes of one view's state planes plus the staged outputs named by mask

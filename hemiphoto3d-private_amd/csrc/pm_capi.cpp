@@ -2,7 +2,7 @@
 //
 // Replaces the per-view driver of runPatchMatchMVS
 // (src/dense_recon/patchmatch_mvs.cu:1028-1258): images live in HBM as
-// plain + row-pair planes, hypothesis state as two colour
+// plain + row-difference planes, hypothesis state as two colour
 // planes, and every launch of a view goes onto the context's stream with no
 // host synchronisation until the outputs are needed (the reference
 // synchronises after every launch, src/utils/cuda_utils.cuh:18-22).
@@ -312,7 +312,7 @@ int hp3d_pm_image_upload(hp3d_pm_ctx* ctx, const void* data, int W, int H, size_
     image_plane_sizes(W, H, store, &img.pitch_plain, &img.plain_bytes, &img.coef_bytes);
     int rc = ctx->dev_alloc(&img.plain, img.plain_bytes, "image plane");
     if (rc) return rc;
-    rc = ctx->dev_alloc(&img.coef, img.coef_bytes, "image row-pair plane");
+    rc = ctx->dev_alloc(&img.coef, img.coef_bytes, "image row-difference plane");
     if (rc) { ctx->dev_free(img.plain, img.plain_bytes); return rc; }
     auto release = [&] {
         ctx->dev_free(img.plain, img.plain_bytes);

@@ -5,8 +5,9 @@
 //   * images: plain[y][x] = DN - 128, row pitch a multiple of 64 elements.
 //     The shift keeps window sums small (the NCC is invariant to it).
 //     Beside the plain plane (reference windows, staged in LDS) every image
-//     has a row-pair plane pair[y][x] = (plain[y][x], plain[y+1][x]) for
-//     source gathers: one load at (x, y) returns the whole bilinear quad.
+//     has a row-difference plane diff[y][x] = (plain[y][x], plain[y+1][x] -
+//     plain[y][x]) for source gathers: one load at (x, y) returns the whole
+//     bilinear quad.
 //     F32 storage: 4 + 8 B/px; F16 storage: 2 + 4 B/px.  For 8-bit input
 //     every value is an integer in [-128, 127], exact in FP16, so F16
 //     storage is lossless and gives the same results as F32.
@@ -50,7 +51,7 @@ struct DevSrc {
 struct DevView {
     const void* ref_plain;              // reference plane
     const void* src_plain[kMaxSrc];     // source planes
-    const void* src_coef[kMaxSrc];      // source gather planes (row pairs)
+    const void* src_coef[kMaxSrc];      // source gather planes (row differences)
     int pitch_plain;                    // elements per row of a plane (both kinds)
     int plain_bytes;                    // bytes of one plane (buffer range check)
     int coef_bytes;                     // bytes of one gather plane

@@ -13,8 +13,8 @@
 //     radius-wide halo, clamped at the image edge) in LDS once;
 //   * per candidate and source the plane homography H = A + b (K^-T n)^T / (n.P)
 //     is formed once (window centre in FP64), so a tap costs 3 FMAs for the
-//     warp, one v_rcp_f32, one buffer load (the row-pair quad) and 6 FMAs
-//     for the bilinear sample;
+//     warp, one v_rcp_f32, one buffer load (the quad from the
+//     row-difference plane) and 4 instructions for the bilinear sample;
 //   * neighbour candidates that provably cannot win are skipped and the rest
 //     shared across the wave's lanes (halfpass_pixel in pm_pixel.h);
 //   * the five NCC moments are accumulated in FP32 on values shifted by
@@ -61,10 +61,12 @@ __global__ __launch_bounds__(TX * TY) void pm_cost_kernel(DevView v, int parity,
     constexpr int R = (NT - 1) * ST / 2;
     using G = TileGeom<R, STORE, TX, TY>;
     __shared__ float tile[G::LH * G::LP];
+    __shared__ WaveXchg xw[TX * TY / 64];
     int bx, by;
     if (!xcd_tile(tiles_x, tiles, bx, by)) return;
     stage_ref_tile<R, STORE, TX, TY>(v, tile, bx, by);
-    cost_pixel<NT, ST, STORE, TX, TY>(v, tile, bx, by, threadIdx.x % TX, threadIdx.x / TX, parity);
+    cost_tile_sorted<NT, ST, STORE, TX, TY>(v, tile, bx, by, threadIdx.x % TX, threadIdx.x / TX,
+                                            parity, xw[threadIdx.x >> 6]);
 }
 
 // Build-time tuning knob of the per-view kernels: minimum waves per SIMD the
@@ -113,8 +115,8 @@ __global__ __launch_bounds__(256) void pm_import_kernel(DevView v, const float* 
 }
 
 // 8-bit or [0,1] float gray -> the image planes (DN - 128, see pm_device.h):
-// the plain plane and the row-pair gather plane (I[y][x], I[y+1][x]), in
-// FP32 or FP16.  in_u8 / in_f32: exactly one is non-null; in_pitch in bytes.
+// the plain plane and the row-difference gather plane
+// (I[y][x], I[y+1][x] - I[y][x]), in FP32 or FP16.  in_u8 / in_f32: exactly one is non-null; in_pitch in bytes.
 template <int STORE>
 __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, const float* in_f32,
                                                        size_t in_pitch, int W, int H, void* plain,
@@ -130,23 +132,26 @@ __global__ __launch_bounds__(256) void img_prep_kernel(const uint8_t* in_u8, con
     };
     const float a = val(x, y);
     const size_t q = (size_t)y * pitch + x;
+    // the last row's difference is never read: taps need v < H - 1
+    const float dn = y < H - 1 ? val(x, y + 1) - a : 0.0f;
     if (STORE == kStoreF32) {
         static_cast<float*>(plain)[q] = a;
-        static_cast<float2*>(coef)[q] = make_float2(a, y < H - 1 ? val(x, y + 1) : 0.0f);
+        static_cast<float2*>(coef)[q] = make_float2(a, dn);
     } else {
         static_cast<_Float16*>(plain)[q] = (_Float16)a;
         half2_t h;
         h.x = (_Float16)a;
-        h.y = (_Float16)(y < H - 1 ? val(x, y + 1) : 0.0f);
+        h.y = (_Float16)dn;
         static_cast<half2_t*>(coef)[q] = h;
     }
 }
 
 // One pyramid level from the previous one: element (x, y) is the mean of
 // the 2 x 2 block at (2x, 2y) of the previous level's plain plane (stored
-// DN - 128 values), written as the level's plain and row-pair planes.  The
-// sums are exact in FP32 (values carry 2 more fraction bits per level); FP16
-// storage is exact for the first two levels.
+// DN - 128 values), written as the level's plain and row-difference planes.
+// The sums are exact in FP32 (values carry 2 more fraction bits per level);
+// FP16 storage holds the values of the first two levels and the row
+// differences of the first level exactly.
 template <int STORE>
 __global__ __launch_bounds__(256) void img_down_kernel(const void* prev, int prev_pitch, int W,
                                                        int H, void* plain, void* coef, int pitch)
@@ -161,17 +166,20 @@ __global__ __launch_bounds__(256) void img_down_kernel(const void* prev, int pre
         const float d = load_plain<STORE>(prev, q + prev_pitch + 1);
         return ((a + b) + (c + d)) * 0.25f;
     };
-    const float a = val(x, y);
-    const float n = y < H - 1 ? val(x, y + 1) : 0.0f;
     const size_t q = (size_t)y * pitch + x;
     if (STORE == kStoreF32) {
+        const float a = val(x, y);
+        const float dn = y < H - 1 ? val(x, y + 1) - a : 0.0f;
         static_cast<float*>(plain)[q] = a;
-        static_cast<float2*>(coef)[q] = make_float2(a, n);
+        static_cast<float2*>(coef)[q] = make_float2(a, dn);
     } else {
-        static_cast<_Float16*>(plain)[q] = (_Float16)a;
+        // the level's stored (FP16) values, and their difference as stored
+        const _Float16 a = (_Float16)val(x, y);
+        const _Float16 b = (_Float16)(y < H - 1 ? val(x, y + 1) : 0.0f);
+        static_cast<_Float16*>(plain)[q] = a;
         half2_t h;
-        h.x = (_Float16)a;
-        h.y = (_Float16)n;
+        h.x = a;
+        h.y = y < H - 1 ? (_Float16)((float)b - (float)a) : (_Float16)0.0f;
         static_cast<half2_t*>(coef)[q] = h;
     }
 }

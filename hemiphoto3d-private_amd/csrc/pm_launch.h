@@ -16,7 +16,7 @@ hipError_t launch_finalize(const DevView& v, float* depth, float* normal, float*
                            hipStream_t st);
 hipError_t launch_import(const DevView& v, const float* depth, const float* normal,
                          hipStream_t st);
-// coef: the row-pair gather plane; both planes use `pitch` elements per row.
+// coef: the row-difference gather plane; both planes use `pitch` elements per row.
 hipError_t launch_img_prep(const uint8_t* in_u8, const float* in_f32, size_t in_pitch, int W,
                            int H, int store, void* plain, void* coef, int pitch, hipStream_t st);
 // Pyramid level (W x H) from the previous level's plain plane (2W x 2H or

@@ -104,6 +104,18 @@ __device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, int b
 // a tile's pass reads what workgroups on other XCDs wrote in the same launch.
 // The image planes are read-only within a launch and keep ordinary cached
 // loads either way.  Planes addressed this way hold less than 2 GiB.
+// How the persistent kernel keeps the state planes coherent across XCDs
+// (build-time choice, measured): bit 0 set = its state stores are ordinary
+// stores followed by an agent-scope release fence (an L2 write-back) before a
+// tile's counter moves, else device-coherent (sc1) stores; bit 1 set = its
+// state loads are ordinary loads after an agent-scope acquire fence (an L2
+// invalidate) once the tile's dependencies are met, else sc1 loads.
+#ifndef HP3D_PERSIST_MEM
+#define HP3D_PERSIST_MEM 0
+#endif
+constexpr bool kPersistPlainStores = (HP3D_PERSIST_MEM & 1) != 0;
+constexpr bool kPersistPlainLoads = (HP3D_PERSIST_MEM & 2) != 0;
+
 template <bool COH> struct StateIO;
 template <> struct StateIO<false> {
     static __device__ __forceinline__ float4 ld4(const float4* p, int i) { return p[i]; }
@@ -122,50 +134,50 @@ template <> struct StateIO<true> {
     {
         return make_rsrc(p, 0x7fffffff);
     }
+    static constexpr int kLd = kPersistPlainLoads ? 0 : kSC1;
+    static constexpr int kSt = kPersistPlainStores ? 0 : kSC1;
     static __device__ __forceinline__ float4 ld4(const float4* p, int i)
     {
         return __builtin_bit_cast(float4,
-                                  __builtin_amdgcn_raw_buffer_load_b128(rs(p), i * 16, 0, kSC1));
+                                  __builtin_amdgcn_raw_buffer_load_b128(rs(p), i * 16, 0, kLd));
     }
     static __device__ __forceinline__ float ld1(const float* p, int i)
     {
-        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs(p), i * 4, 0, kSC1));
+        return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(rs(p), i * 4, 0, kLd));
     }
     static __device__ __forceinline__ unsigned ldb(const unsigned char* p, int i)
     {
-        return __builtin_amdgcn_raw_buffer_load_b8(rs(p), i, 0, kSC1);
+        return __builtin_amdgcn_raw_buffer_load_b8(rs(p), i, 0, kLd);
     }
     static __device__ __forceinline__ void st4(float4* p, int i, float4 x)
     {
         __builtin_amdgcn_raw_buffer_store_b128(__builtin_bit_cast(__attribute__((ext_vector_type(4))) unsigned, x),
-                                               rs(p), i * 16, 0, kSC1);
+                                               rs(p), i * 16, 0, kSt);
     }
     static __device__ __forceinline__ void st1(float* p, int i, float x)
     {
-        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), rs(p), i * 4, 0, kSC1);
+        __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(unsigned, x), rs(p), i * 4, 0, kSt);
     }
     static __device__ __forceinline__ void stb(unsigned char* p, int i, unsigned x)
     {
-        __builtin_amdgcn_raw_buffer_store_b8((unsigned char)x, rs(p), i, 0, kSC1);
+        __builtin_amdgcn_raw_buffer_store_b8((unsigned char)x, rs(p), i, 0, kSt);
     }
 };
 
-// a (1 - t) + b t as two FMAs.
-__device__ __forceinline__ float lerp_fma(float t, float a, float b)
-{
-    return fmaf(t, b, fmaf(-t, a, a));
-}
-
 // Bilinear sample at (ix + au, iy + aw) of the texel quad a = I[iy][ix],
 // b = I[iy][ix+1], c = I[iy+1][ix], d = I[iy+1][ix+1].  Both storages keep a
-// row-pair plane for source gathers, element (x, y) = (I[y][x], I[y+1][x]),
-// so that one load per tap at (ix, iy) returns the whole quad as (a, c, b, d):
-// 16 bytes from 8 bytes per pixel (F32) or 8 bytes from 4 (F16).  The sample
-// is two row lerps and a column lerp in FP32 either way; 8-bit gray levels
-// are exact in FP16, so for 8-bit images both storages give identical results.
-// (Measured on 1600x1200 views: one load per tap instead of one per texel row
-// is worth 1.6x at stride 2; the F16 plane's halved footprint 1.13x more at
-// stride 1.)
+// row-difference plane for source gathers, element (x, y) =
+// (I[y][x], I[y+1][x] - I[y][x]), so that one load per tap at (ix, iy)
+// returns the whole quad as (a, c - a, b, d - b): 16 bytes from 8 bytes per
+// pixel (F32) or 8 bytes from 4 (F16).  The sample is then two FMAs down the
+// columns and a subtract and an FMA across -- 4 VALU instructions, against 6
+// for three lerps of a row-pair plane (a, c, b, d) -- in FP32 either way.
+// For 8-bit images the values (DN - 128) and the row differences (at most
+// 255 in magnitude) are integers, exact in FP16, so both storages give
+// identical results.  (Measured on 1600x1200 views: one load per tap instead
+// of one per texel row, 1.6x at stride 2; the F16 plane's halved footprint
+// 1.13x more at stride 1; differences instead of row pairs, +2 % (F32) /
+// +5 % (F16) at stride 2 and +16 % at stride 1.)
 template <int STORE> struct SrcImage;
 template <> struct SrcImage<kStoreF32> {
     __amdgpu_buffer_rsrc_t rsrc;
@@ -177,7 +189,8 @@ template <> struct SrcImage<kStoreF32> {
         unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 3;
         off = ok ? off : kOffNowhere;
         const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
-        return lerp_fma(aw, lerp_fma(au, q.x, q.z), lerp_fma(au, q.y, q.w));
+        const float c0 = fmaf(aw, q.y, q.x), c1 = fmaf(aw, q.w, q.z);
+        return fmaf(au, c1 - c0, c0);
     }
 };
 template <> struct SrcImage<kStoreF16> {
@@ -190,7 +203,8 @@ template <> struct SrcImage<kStoreF16> {
         unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 2;
         off = ok ? off : kOffNowhere;
         const half4_t q = __builtin_bit_cast(half4_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
-        return lerp_fma(aw, lerp_fma(au, (float)q.x, (float)q.z), lerp_fma(au, (float)q.y, (float)q.w));
+        const float c0 = fmaf(aw, (float)q.y, (float)q.x), c1 = fmaf(aw, (float)q.w, (float)q.z);
+        return fmaf(au, c1 - c0, c0);
     }
 };
 
@@ -530,10 +544,77 @@ struct WaveXchg {
     unsigned char item[256];
     float4 hyp[256];
     float cost[256];
-#if HP3D_BUCKET
     int hist[64];
-#endif
 };
+
+// Cost of the current hypotheses of colour `parity` at this workgroup's
+// pixels (compute_ncc_cost_kernel, patchmatch_mvs.cu:349-513), evaluated in
+// inverse-depth order within each wave: the hypotheses are random, so each
+// lane's windows would gather from an unrelated part of every source; a
+// counting sort of the wave's hypotheses by inverse depth (disparity) hands
+// lane k the k-th, so a wave's gathers fall in a narrower band of each
+// source.  Any lane can evaluate any pixel's candidate (the whole window
+// tile is in LDS) and a cost is a pure function of the pixel and the
+// hypothesis, so the result is the same as one lane per pixel.  Every
+// thread of the workgroup must call this (it synchronises the group).
+template <int NT, int ST, int STORE, int TX, int TY, bool COH = false>
+__device__ __forceinline__ void cost_tile_sorted(const DevView& v, const float* tile, int bx,
+                                                 int by, int tx, int ty, int parity, WaveXchg& xw)
+{
+    constexpr int R = (NT - 1) * ST / 2;
+    using G = TileGeom<R, STORE, TX, TY>;
+    using IO = StateIO<COH>;
+    constexpr int kBuckets = 64;
+    const int lane = threadIdx.x & 63;
+    const int wave0 = threadIdx.x & ~63;
+    const int i = bx * TX + tx, y = by * TY + ty;
+    const int x = 2 * i + ((y + parity) & 1);
+    const bool active = y < v.H && x < v.W;
+    float4 h = make_float4(0.f, 0.f, 0.f, 0.f);
+    if (active) h = IO::ld4(v.hyp[parity], y * v.Wc + i);
+    xw.hist[lane] = 0;
+    __builtin_amdgcn_wave_barrier();
+    int bk = 0, rk = 0;
+    if (active) {
+        const float ilo = 1.0f / v.dmax, iscale = (float)kBuckets / (1.0f / v.dmin - ilo);
+        const int b = (int)((1.0f / h.x - ilo) * iscale);
+        bk = b < 0 ? 0 : (b > kBuckets - 1 ? kBuckets - 1 : b);
+        rk = atomicAdd(&xw.hist[bk], 1);
+    }
+    __builtin_amdgcn_wave_barrier();
+    const int cnt = xw.hist[lane];
+    int incl = cnt;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+        const int t = __shfl_up(incl, d, 64);
+        if (lane >= d) incl += t;
+    }
+    __builtin_amdgcn_wave_barrier();
+    xw.hist[lane] = incl - cnt;
+    __builtin_amdgcn_wave_barrier();
+    if (active) {
+        const int slot = xw.hist[bk] + rk;
+        xw.item[slot] = (unsigned char)lane;
+        xw.hyp[slot] = h;
+    }
+    const unsigned total = (unsigned)__popcll(__ballot(active));
+    __syncthreads();
+    if ((unsigned)lane < total) {
+        const int ol = xw.item[lane];
+        const float4 hh = xw.hyp[lane];
+        const int t_o = wave0 + ol;
+        const int tx_o = t_o % TX, ty_o = t_o / TX;
+        const int i_o = bx * TX + tx_o, y_o = by * TY + ty_o;
+        const int x_o = 2 * i_o + ((y_o + parity) & 1);
+        const float* lwin = tile + ty_o * G::LP + (x_o - 2 * bx * TX);
+        const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
+        double rx, ry, rz;
+        ray_fp64(v.kinv_d, x_o, y_o, rx, ry, rz);
+        IO::st1(v.cost[parity], y_o * v.Wc + i_o,
+                cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x_o, y_o, rx, ry, rz, hh.x, hh.y,
+                                         hh.z, hh.w));
+    }
+}
 
 // One pixel of a red (parity 0) or black (parity 1) half-pass:
 // propagate_and_refine_kernel (:519-655).  Candidates L, R, U, D from the

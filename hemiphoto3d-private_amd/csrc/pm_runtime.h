@@ -13,7 +13,7 @@
 
 namespace hp3d {
 
-// One uploaded image: its plane of DN - 128 values and the row-pair gather
+// One uploaded image: its plane of DN - 128 values and the row-difference gather
 // plane of source taps (pm_pixel.h).
 struct Image {
     int W = 0, H = 0, store = 0;
@@ -30,7 +30,7 @@ inline void image_plane_sizes(int W, int H, int store, int* pitch, size_t* plain
 {
     *pitch = (W + 63) & ~63;
     *plain_bytes = (size_t)*pitch * H * (store == HP3D_STORE_F32 ? 4 : 2);
-    *coef_bytes = (size_t)*pitch * H * (store == HP3D_STORE_F32 ? 8 : 4);   // row pairs
+    *coef_bytes = (size_t)*pitch * H * (store == HP3D_STORE_F32 ? 8 : 4);   // row differences
 }
 
 // True if a W x H image with this storage can be addressed by the source

@@ -35,8 +35,9 @@ struct StepItem {
     int done_base;     // persistent kernel: first of the view's tile counters
 };
 
-// Persistent batch kernel: control words after the per-tile counters.
-enum PersistCtl : int { kCtlNextChunk = 0, kCtlAbort = 1, kCtlWords = 2 };
+// Persistent batch kernel: control words after the per-tile counters -- the
+// next chunk of each of the 8 per-XCD work queues, and the abort flag.
+enum PersistCtl : int { kCtlQueue0 = 0, kCtlAbort = 8, kCtlWords = 9 };
 
 struct StepVariant {
     int tx, ty;        // compacted columns x rows per workgroup

@@ -158,6 +158,12 @@ std::vector<DepthMap> runPatchMatchMVS(const std::vector<ImageData>& images,
     std::vector<DepthMap> maps;
     std::vector<const MVSViewParams*> map_params;
     maps.reserve(view_params.size());
+    // every view's descriptor first, then one call that solves them back to
+    // back with each view's download overlapping the next view's solve
+    std::vector<hp3d_view_desc> descs;
+    std::vector<std::vector<hp3d_seed_point>> seed_sets;
+    descs.reserve(view_params.size());
+    seed_sets.reserve(view_params.size());
     for (size_t vi = 0; vi < view_params.size(); ++vi) {
         const MVSViewParams& vp = view_params[vi];
         const int ref_id = vp.ref_image_id;
@@ -202,7 +208,8 @@ std::vector<DepthMap> runPatchMatchMVS(const std::vector<ImageData>& images,
 
         // SfM seeds of the reference camera (cameras[], as the reference's driver uses)
         const hp3d_sfm_camera cref = to_c(rc, ref_id);
-        std::vector<hp3d_seed_point> seeds(std::max<size_t>(1, pts.pts.size()));
+        seed_sets.emplace_back(std::max<size_t>(1, pts.pts.size()));
+        std::vector<hp3d_seed_point>& seeds = seed_sets.back();
         int n_seeds = 0;
         check(hp3d_project_seed_points(&cref, pts.pts.data(), (int)pts.pts.size(), vp.depth_min,
                                        vp.depth_max, w, h, kSeedRadius, seeds.data(), &n_seeds),
@@ -210,6 +217,7 @@ std::vector<DepthMap> runPatchMatchMVS(const std::vector<ImageData>& images,
         d.seeds = seeds.data();
         d.num_seeds = n_seeds;
         d.seed_radius = kSeedRadius;
+        descs.push_back(d);
 
         DepthMap dm;
         dm.image_id = ref_id;
@@ -218,17 +226,23 @@ std::vector<DepthMap> runPatchMatchMVS(const std::vector<ImageData>& images,
         dm.depth.resize((size_t)w * h);
         dm.normal.resize((size_t)w * h * 3);
         dm.confidence.resize((size_t)w * h);
-        hp3d_view_out out{};
-        out.depth = dm.depth.data();
-        out.normal = dm.normal.data();
-        out.confidence = dm.confidence.data();
-        check(hp3d_pm_run_view(ctx.get(), &d, &out), ctx.get(), "hp3d_pm_run_view");
-        if (config.save_intermediate && !config.output_path.empty())
-            saveDepthMapPFM(dm, dir_of(config.output_path) + "/depth_" + std::to_string(ref_id) +
-                                    ".pfm");
         maps.push_back(std::move(dm));
         map_params.push_back(&vp);
     }
+    if (!maps.empty()) {
+        std::vector<hp3d_view_out> outs(maps.size());
+        for (size_t i = 0; i < maps.size(); ++i) {
+            outs[i].depth = maps[i].depth.data();
+            outs[i].normal = maps[i].normal.data();
+            outs[i].confidence = maps[i].confidence.data();
+        }
+        check(hp3d_pm_run_views(ctx.get(), descs.data(), (int)descs.size(), outs.data()),
+              ctx.get(), "hp3d_pm_run_views");
+    }
+    if (config.save_intermediate && !config.output_path.empty())
+        for (const DepthMap& dm : maps)
+            saveDepthMapPFM(dm, dir_of(config.output_path) + "/depth_" +
+                                    std::to_string(dm.image_id) + ".pfm");
     for (hp3d_img hnd : handle)
         if (hnd) hp3d_pm_image_release(ctx.get(), hnd);
 

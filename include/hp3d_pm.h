@@ -258,7 +258,7 @@ typedef struct {
                         of (view, step, tile) units, each half-pass of a tile starting as
                         soon as that tile and its 4 neighbours finished the previous pass
                         (per-tile counters), instead of one launch per step */
-    int chunk;       /* persistent: units a workgroup takes at a time (0 = 8) */
+    int chunk;       /* persistent: units a workgroup takes at a time (0 = 16) */
 } hp3d_batch_opts;
 
 int hp3d_pm_batch_run(hp3d_pm_ctx* ctx, const hp3d_view_desc* descs, int n,

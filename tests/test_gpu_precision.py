@@ -10,8 +10,10 @@ them is the same as with FP32 storage.  Gates (tolerances written in the tests):
   * full run: FP16 depth within 1 % of the FP32-storage depth on >= 99 % of
     pixels, and its ground-truth accuracy no worse than FP32 storage's (in
     fact the runs are bit-identical, which the test also asserts).
-The oracle's full schedule at this size takes minutes of CPU, so depth parity
-against the oracle itself is tested at smaller sizes in test_gpu_parity.py.
+  * depth of the full solves against the oracle's FP64-cost run: >= 99 % of
+    20000 seeded pixels within 1 % (the oracle's run at this size takes
+    minutes of CPU, so its depths at those pixels are a committed fixture,
+    tests/golden/c3_fp64_depth_sample.npz, made by make_c3_depth_sample.py).
 """
 import numpy as np
 import pytest
@@ -88,3 +90,36 @@ def test_c3_fp16_run_matches_fp32_storage(pkg, ctx, c3):
     assert np.array_equal(o16, o32)
     assert np.all((o16 >= job.dmin) & (o16 <= job.dmax))
     _release(ctx, h32 + h16)
+
+
+def test_c3_depth_against_fp64_oracle(pkg, ctx, c3):
+    """Depth error of the full-size solves (FP32 and FP16 storage) against the
+    oracle's FP64-cost run of the same view, at the 20000 seeded pixels of
+    tests/golden/c3_fp64_depth_sample.npz (made by make_c3_depth_sample.py:
+    the oracle's run takes minutes of CPU).  Gate: >= 99 % of the sampled
+    pixels within 1 % of the oracle's depth, ground-truth accuracy no worse
+    than the oracle's less 0.3 %; the depth error is printed."""
+    import hashlib
+    import os
+    g = np.load(os.path.join(os.path.dirname(__file__), "golden", "c3_fp64_depth_sample.npz"))
+    job = c3
+    h = hashlib.sha256()
+    for im in [job.ref] + list(job.srcs):
+        h.update(np.ascontiguousarray(im).tobytes())
+    assert h.hexdigest() == str(g["inputs_sha256"]), "rendered inputs differ from the fixture's"
+    assert tuple(g["shape"]) == job.shape and int(g["iterations"]) == job.iterations
+    idx, od = g["index"], g["depth"].astype(np.float64)
+    gt = job.gt_depth.reshape(-1)[idx]
+    for store in (0, 1):
+        desc, handles = _upload(pkg, ctx, job, store, job.iterations)
+        out = ctx.run_view(desc, job.shape, with_normal=False, with_cost=False)["depth"]
+        _release(ctx, handles)
+        d = out.reshape(-1)[idx].astype(np.float64)
+        rel = np.abs(d - od) / np.abs(od)
+        within = float((rel <= 0.01).mean())
+        acc = float((np.abs(d - gt) <= 0.01 * np.abs(gt)).mean())
+        print(f"c3 store={store}: within 1 % of the FP64 oracle on {within:.5f} of the sample, "
+              f"mean relative depth error {rel.mean():.2e}, median {np.median(rel):.2e}; "
+              f"GT within 1 %: gpu {acc:.4f} oracle {float(g['gt_within_1pct']):.4f}")
+        assert within >= 0.99
+        assert acc >= float(g["gt_within_1pct"]) - 0.003
