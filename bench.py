@@ -22,6 +22,8 @@ roofline: the dominant kernel is the half-pass (pm_halfpass_kernel).
   with HIP events recorded by the library on its own stream during the timed
   steps.  Peak: 157.3 TFLOP/s (MI355X FP32 vector, datasheet).  The
   kernel issues no MFMA and is bound by VALU issue, so bound = "valu".
+  achieved_executed: the same over the taps the kernel executes (counter
+  SQ_INSTS_VALU_TRANS_F32, one v_rcp_f32 per tap, x 64 lanes).
   traffic, hbm_gbs, l2_read_gbs: HBM bytes (FETCH_SIZE + WRITE_SIZE) and
   L2-to-L1 read bytes (TCP_TCC_READ_REQ) per launch from rocprofv3 counter
   runs of this workload, calibrated on a 1 GiB copy and kept in
@@ -180,6 +182,15 @@ def main():
     if traffic:
         gbs = traffic / avg_launch_s / 1e9
         roofline.update(hbm_gbs=round(gbs, 1), hbm_frac=round(gbs / PEAK_HBM_GBS, 4))
+    if prof.get("executed_taps_per_launch"):
+        # the taps the kernel executes (one v_rcp_f32 each, counted by
+        # SQ_INSTS_VALU_TRANS_F32 x 64 lanes; candidates it proves rejected
+        # are not evaluated), beside the algorithmic taps above
+        ex = prof["executed_taps_per_launch"] * FLOP_PER_TAP / avg_launch_s / 1e12
+        roofline.update(executed_taps_per_launch=prof["executed_taps_per_launch"],
+                        algorithmic_taps_per_launch=taps_per_launch,
+                        achieved_executed=round(ex, 3),
+                        frac_executed=round(ex / PEAK_FP32_TFLOPS, 4))
     if prof.get("l2_read_bytes_per_launch"):
         l2 = prof["l2_read_bytes_per_launch"] / avg_launch_s / 1e9
         roofline.update(l2_read_gbs=round(l2, 1), l2_frac=round(l2 / prof["l2_peak_gbs"], 4),
