@@ -16,6 +16,7 @@
 #include <vector>
 
 #include "pm_runtime.h"
+#include "pm_xcd.h"
 
 namespace hp3d {
 
@@ -35,10 +36,14 @@ struct GeomViewDev {
     GeomSrcDev src[kMaxSrc];
 };
 
-__global__ __launch_bounds__(256) void geom_filter_kernel(GeomViewDev g)
+// One 64x4-pixel tile per workgroup, tiles in XCD order: the source maps'
+// rows that a band of the view projects to stay in one XCD's L2.
+__global__ __launch_bounds__(256) void geom_filter_kernel(GeomViewDev g, int tiles_x, int tiles)
 {
-    const int x = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
+    int bx, by;
+    if (!xcd_tile(tiles_x, tiles, bx, by)) return;
+    const int x = bx * 64 + (threadIdx.x & 63);
+    const int y = by * 4 + (threadIdx.x >> 6);
     if (x >= g.W || y >= g.H) return;
     const size_t idx = (size_t)y * g.W + x;
     const float d = g.depth[idx];
@@ -178,8 +183,9 @@ extern "C" int hp3d_pm_geom_filter(hp3d_pm_ctx* ctx, hp3d_geom_view* views, int 
         g.depth = dptr(i, 0);
         g.normal = dptr(i, 1);
         g.conf = v.confidence ? dptr(i, 4) : nullptr;
-        hipLaunchKernelGGL(geom_filter_kernel, dim3((v.width + 63) / 64, (v.height + 3) / 4),
-                           dim3(256), 0, ctx->stream, g);
+        const int tiles_x = (v.width + 63) / 64, tiles = tiles_x * ((v.height + 3) / 4);
+        hipLaunchKernelGGL(geom_filter_kernel, dim3(xcd_grid(tiles)), dim3(256), 0, ctx->stream, g,
+                           tiles_x, tiles);
         e = hipGetLastError();
         filtered[i] = 1;
     }

@@ -25,6 +25,7 @@
 #include "pm_device.h"
 #include "pm_launch.h"
 #include "pm_pixel.h"
+#include "pm_xcd.h"
 
 namespace hp3d {
 
@@ -34,22 +35,6 @@ __global__ __launch_bounds__(256) void pm_init_kernel(DevView v, uint64_t seed)
     const int y = blockIdx.y * 4 + (threadIdx.x >> 6);
     if (x >= v.W || y >= v.H) return;
     v.hyp[(x + y) & 1][y * v.Wc + (x >> 1)] = init_hyp(v, x, y, seed);
-}
-
-// Tile of this workgroup.  The grid is 1-D, a multiple of 8 blocks, and
-// workgroups are dispatched to the 8 XCDs round-robin (block b -> XCD b % 8):
-// block b takes tile (b % 8) * (nblocks / 8) + b / 8 in row-major tile order,
-// so each XCD sweeps a contiguous band of the image and neighbouring tiles,
-// which gather from the same source rows, share that XCD's L2.  Returns
-// false for the padding blocks.
-__device__ __forceinline__ bool xcd_tile(int tiles_x, int tiles, int& bx, int& by)
-{
-    const int nb = gridDim.x, b = blockIdx.x;
-    const int t = (b & 7) * (nb >> 3) + (b >> 3);
-    if (t >= tiles) return false;
-    by = t / tiles_x;
-    bx = t - by * tiles_x;
-    return true;
 }
 
 // Cost of the current hypotheses of one colour, run once per colour before

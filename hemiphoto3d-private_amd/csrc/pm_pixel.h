@@ -780,7 +780,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
     for (unsigned r = 0; r < total; r += 64) {
         const unsigned g = r + lane;
         const unsigned it = xw.item[g < total ? g : total - 1];
-        const int ol = it & 63, k = it >> 6;
+        const int ol = it & 63;
         const WinRef wro{__shfl(wr.sr, ol, 64), __shfl(wr.srr, ol, 64)};
         if (g < total) {
             const int t_o = wave0 + ol;

@@ -34,6 +34,7 @@ class OrcView(ctypes.Structure):
         ("b", (ctypes.c_double * 3) * MAX_SRC),
         ("dmin", ctypes.c_float), ("dmax", ctypes.c_float),
         ("radius", ctypes.c_int), ("stride", ctypes.c_int), ("top_k", ctypes.c_int),
+        ("ref_gray", _f32p), ("src_gray", _f32p * MAX_SRC),
     ]
 
 
@@ -170,13 +171,26 @@ def compose(Kr, Rr, tr, Ks, Rs, ts):
 class View:
     """A reference view with its sources, as the oracle sees it.
 
-    ref, srcs: uint8 arrays (H, W).  Kinv: (3,3) float64.  A: list of (3,3),
-    b: list of (3,) float64 -- the FP64-composed homography terms.
+    ref, srcs: uint8 arrays (H, W) of gray levels (DN, gray = DN / 255), or
+    float arrays of gray itself (the reference's float images, nominally in
+    [0, 1]; held as float32, as the reference holds them).  Kinv: (3,3)
+    float64.  A: list of (3,3), b: list of (3,) float64 -- the FP64-composed
+    homography terms.
     """
 
     def __init__(self, ref, srcs, Kinv, A, b, dmin, dmax, radius=5, stride=1, top_k=3):
-        self.ref = np.ascontiguousarray(ref, dtype=np.uint8)
-        self.srcs = [np.ascontiguousarray(s, dtype=np.uint8) for s in srcs]
+        is_float = np.issubdtype(np.asarray(ref).dtype, np.floating)
+        assert all(np.issubdtype(np.asarray(s).dtype, np.floating) == is_float for s in srcs)
+        if is_float:
+            self.gray = [np.ascontiguousarray(ref, dtype=np.float32)] + \
+                [np.ascontiguousarray(s, dtype=np.float32) for s in srcs]
+            # the 8-bit planes are not read when the float ones are set
+            self.ref = np.zeros(self.gray[0].shape, np.uint8)
+            self.srcs = [self.ref] * len(srcs)
+        else:
+            self.gray = None
+            self.ref = np.ascontiguousarray(ref, dtype=np.uint8)
+            self.srcs = [np.ascontiguousarray(s, dtype=np.uint8) for s in srcs]
         H, W = self.ref.shape
         for s in self.srcs:
             assert s.shape == (H, W)
@@ -187,6 +201,12 @@ class View:
         v.ref = _p(self.ref, _u8p)
         for i, s in enumerate(self.srcs):
             v.src[i] = _p(s, _u8p)
+        if self.gray is not None:
+            for g in self.gray:
+                assert g.shape == (H, W)
+            v.ref_gray = _p(self.gray[0], _f32p)
+            for i, g in enumerate(self.gray[1:]):
+                v.src_gray[i] = _p(g, _f32p)
         Ki = np.asarray(Kinv, dtype=np.float64).reshape(9)
         for i in range(9):
             v.Kinv[i] = Ki[i]

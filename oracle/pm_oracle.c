@@ -156,14 +156,15 @@ double orc_cost_fp64(const orc_view* v, int x, int y, float d, float nxf, float 
         const double* A = v->A[s];
         const double* b = v->b[s];
         const uint8_t* img = v->src[s];
+        const float* imgf = v->src_gray[s];
         double sr = 0, ss = 0, srr = 0, sss = 0, srs = 0;
         int count = 0;
         for (int j = 0; j < n; ++j) {
             const int dy = -v->radius + v->stride * j;
             for (int i = 0; i < n; ++i) {
                 const int dx = -v->radius + v->stride * i;
-                const double rv =
-                    v->ref[clampi(y + dy, 0, H - 1) * W + clampi(x + dx, 0, W - 1)] / 255.0;
+                const size_t ri = (size_t)clampi(y + dy, 0, H - 1) * W + clampi(x + dx, 0, W - 1);
+                const double rv = v->ref_gray ? (double)v->ref_gray[ri] : v->ref[ri] / 255.0;
                 const double px = fx + dx, py = fy + dy;
                 const double qx = A[0] * px + A[1] * py + A[2];
                 const double qy = A[3] * px + A[4] * py + A[5];
@@ -182,10 +183,15 @@ double orc_cost_fp64(const orc_view* v, int x, int y, float d, float nxf, float 
                 if (u < 0.0 || u >= umax || w < 0.0 || w >= vmax) continue;
                 const int x0 = (int)floor(u), y0 = (int)floor(w);
                 const double ax = u - x0, ay = w - y0;
-                const uint8_t* row0 = img + (size_t)y0 * W + x0;
-                const uint8_t* row1 = row0 + W;
-                const double sv = ((1 - ax) * (1 - ay) * row0[0] + ax * (1 - ay) * row0[1] +
-                                   (1 - ax) * ay * row1[0] + ax * ay * row1[1]) / 255.0;
+                const size_t q0 = (size_t)y0 * W + x0, q1 = q0 + W;
+                double sv;
+                if (imgf) {
+                    sv = (1 - ax) * (1 - ay) * (double)imgf[q0] + ax * (1 - ay) * (double)imgf[q0 + 1] +
+                         (1 - ax) * ay * (double)imgf[q1] + ax * ay * (double)imgf[q1 + 1];
+                } else {
+                    sv = ((1 - ax) * (1 - ay) * img[q0] + ax * (1 - ay) * img[q0 + 1] +
+                          (1 - ax) * ay * img[q1] + ax * ay * img[q1 + 1]) / 255.0;
+                }
                 sr += rv; ss += sv; srr += rv * rv; sss += sv * sv; srs += rv * sv;
                 ++count;
             }
@@ -441,10 +447,11 @@ static float cost_fp32_ref(const f32_view* fv, int x, int y, float cand_depth, f
     return sum / (float)k;
 }
 
-static float* gray_plane(const uint8_t* dn, size_t n)
+static float* gray_plane(const uint8_t* dn, const float* gray, size_t n)
 {
     float* g = (float*)malloc(n * sizeof(float));
-    for (size_t i = 0; i < n; ++i) g[i] = gray_of(dn[i]);
+    if (gray) memcpy(g, gray, n * sizeof(float));
+    else for (size_t i = 0; i < n; ++i) g[i] = gray_of(dn[i]);
     return g;
 }
 
@@ -453,9 +460,9 @@ static void f32_view_open(f32_view* fv, const orc_view* v)
     const size_t n = (size_t)v->W * v->H;
     fv->v = v;
     fv->cost_mode = ORC_COST_FP32;
-    fv->ref = gray_plane(v->ref, n);
+    fv->ref = gray_plane(v->ref, v->ref_gray, n);
     for (int s = 0; s < v->S; ++s) {
-        fv->src[s] = gray_plane(v->src[s], n);
+        fv->src[s] = gray_plane(v->src[s], v->src_gray[s], n);
         for (int i = 0; i < 9; ++i) fv->A[s][i] = (float)v->A[s][i];
         for (int i = 0; i < 3; ++i) fv->b[s][i] = (float)v->b[s][i];
     }

@@ -59,6 +59,11 @@ typedef struct {
     int radius;                     /* window half size, 1..5 */
     int stride;                     /* window stride, 1 or 2 */
     int top_k;                      /* reference NCC_TOP_K = 3 */
+    /* Optional float images (gray in the reference's own units, nominally
+     * [0, 1]; NULL: gray = DN / 255 from ref / src).  When set they replace
+     * the 8-bit planes everywhere the oracle reads an image. */
+    const float* ref_gray;
+    const float* src_gray[ORC_MAX_SRC];
 } orc_view;
 
 /* Flags returned by orc_cost_fp64: the result sits near a discrete threshold,

@@ -1,7 +1,7 @@
 """Precision form at full size: 3200x2400 reference, 6 sources at +-5/10/15 deg,
 11x11 window at stride 2, 8 iterations (run on an MI355X).
 
-Storage is FP16 (image plane and row-pair gather plane), accumulation FP32,
+Storage is FP16 (image plane and row-difference gather plane), accumulation FP32,
 homography composition FP64 on the host.  8-bit gray levels shifted by -128 DN
 are small integers, so FP16 holds them exactly and the FP32 arithmetic on
 them is the same as with FP32 storage.  Gates (tolerances written in the tests):
@@ -14,6 +14,10 @@ them is the same as with FP32 storage.  Gates (tolerances written in the tests):
     20000 seeded pixels within 1 % (the oracle's run at this size takes
     minutes of CPU, so its depths at those pixels are a committed fixture,
     tests/golden/c3_fp64_depth_sample.npz, made by make_c3_depth_sample.py).
+  * float images (the input FP16 storage rounds): cost against the FP64
+    oracle on the unrounded images, FP32 storage <= 1e-4, FP16 storage within
+    its rounding (99.9th percentile <= 2e-3, median <= 2e-4); FP16 depth within
+    1 % of FP32 storage's on >= 99 % of pixels, GT accuracy no worse less 0.2 %.
 """
 import numpy as np
 import pytest
@@ -123,3 +127,82 @@ def test_c3_depth_against_fp64_oracle(pkg, ctx, c3):
               f"GT within 1 %: gpu {acc:.4f} oracle {float(g['gt_within_1pct']):.4f}")
         assert within >= 0.99
         assert acc >= float(g["gt_within_1pct"]) - 0.003
+
+
+# ---------------------------------------------------------------------------
+# Float images: the one input FP16 storage rounds.  8-bit levels are exact in
+# FP16, float gray levels are not: DN - 128 in [-128, 128) is held to 2^-5 DN
+# (half an ulp of 2^-4 above 64 DN), its row difference (|.| < 256) to 2^-4 DN.
+# ---------------------------------------------------------------------------
+F16_FLOAT_COST_P999 = 2e-3     # 99.9th percentile of |cost - FP64| with FP16 storage
+F16_FLOAT_COST_MEDIAN = 2e-4
+
+
+@pytest.fixture(scope="module")
+def c3_float(c3):
+    """The C3 images as float gray with sub-DN detail: (DN + u) / 256, u
+    uniform in [0, 1) per pixel (seeded), float32."""
+    rng = np.random.default_rng(77)
+    to_f = lambda a: ((a.astype(np.float64) + rng.uniform(0.0, 1.0, a.shape)) / 256.0).astype(np.float32)
+    return to_f(c3.ref), [to_f(s) for s in c3.srcs]
+
+
+def _upload_float(pkg, ctx, job, imgs, store, iterations):
+    ref = ctx.upload(imgs[0], store)
+    srcs = [ctx.upload(s, store) for s in imgs[1]]
+    desc = pkg.make_desc(ref, srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                         radius=job.radius, stride=job.stride, iterations=iterations,
+                         top_k=job.top_k, seed=job.seed)
+    return desc, [ref] + srcs
+
+
+def test_c3_float_images_cost_against_fp64_oracle(pkg, orc, ctx, c3, c3_float):
+    """Cost of identical hypotheses against the FP64 oracle evaluated on the
+    unrounded float images: FP32 storage within the plain 1e-4 on every
+    unflagged pixel; FP16 storage within its rounding (99.9th percentile and
+    median gates above).  Both errors are printed."""
+    job = c3
+    H, W = job.shape
+    v = orc.View(c3_float[0], c3_float[1], job.Kinv, job.A, job.b, job.dmin, job.dmax,
+                 radius=job.radius, stride=job.stride, top_k=job.top_k)
+    rng = np.random.default_rng(5)
+    ys = rng.integers(0, H, SAMPLE).astype(np.int32)
+    xs = rng.integers(0, W, SAMPLE).astype(np.int32)
+    gt_d = np.where(job.gt_depth > 0, job.gt_depth, job.dmax).astype(np.float32)
+    gt_n = job.gt_normal.copy()
+    gt_n[job.gt_depth <= 0] = (0, 0, -1)
+    for store in (0, 1):
+        desc, handles = _upload_float(pkg, ctx, job, c3_float, store, job.iterations)
+        d0, n0 = ctx.init_hypotheses(desc, job.shape)
+        for name, (d, n) in {"random": (d0, n0), "near-gt": (gt_d, gt_n)}.items():
+            c_gpu = ctx.cost_eval(desc, d, n)[ys, xs].astype(np.float64)
+            c64, flags = v.cost_list_fp64(xs, ys, d[ys, xs], n[ys, xs])
+            ok = flags == 0
+            err = np.abs(c_gpu - c64)[ok]
+            print(f"c3 float images store={store} {name}: max|err| {err.max():.2e} "
+                  f"p99.9 {np.quantile(err, 0.999):.2e} median {np.median(err):.2e}")
+            assert ok.mean() > 0.9
+            if store == 0:
+                assert err.max() <= COST_TOL
+            else:
+                assert np.quantile(err, 0.999) <= F16_FLOAT_COST_P999
+                assert np.median(err) <= F16_FLOAT_COST_MEDIAN
+        _release(ctx, handles)
+
+
+def test_c3_float_images_fp16_depth(pkg, ctx, c3, c3_float):
+    """Full solves on the float images: FP16 storage's depth within 1 % of
+    FP32 storage's on >= 99 % of pixels, ground-truth accuracy no worse than
+    FP32 storage's less 0.2 %."""
+    job = c3
+    out = {}
+    for store in (0, 1):
+        desc, handles = _upload_float(pkg, ctx, job, c3_float, store, job.iterations)
+        out[store] = ctx.run_view(desc, job.shape, with_normal=False, with_cost=False)["depth"]
+        _release(ctx, handles)
+    agree = rel_within(out[1], out[0]).mean()
+    valid = job.gt_depth > 0
+    acc = {s: rel_within(out[s][valid], job.gt_depth[valid]).mean() for s in (0, 1)}
+    print(f"c3 float images: fp16~fp32 {agree:.5f} | GT<1%: fp32 {acc[0]:.4f} fp16 {acc[1]:.4f}")
+    assert agree >= 0.99
+    assert acc[1] >= acc[0] - 0.002

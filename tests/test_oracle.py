@@ -226,8 +226,10 @@ def brute_cost(ref, srcs, K, cams, x, y, d, n, radius, stride, top_k):
     offs = -radius + stride * np.arange(2 * radius // stride + 1)
     k = len(offs) ** 2
     costs = []
+    # 8-bit images hold DN (gray = DN / 255), float images gray itself
+    unit = 1.0 if np.issubdtype(ref.dtype, np.floating) else 255.0
     for Rs, ts in cams[1:]:
-        src = srcs[len(costs)].astype(np.float64) / 255.0
+        src = srcs[len(costs)].astype(np.float64) / unit
         rs, ss = [], []
         for dy in offs:
             for dx in offs:
@@ -246,7 +248,7 @@ def brute_cost(ref, srcs, K, cams, x, y, d, n, radius, stride, top_k):
                 ax, ay = u - x0, v - y0
                 s = ((1 - ax) * (1 - ay) * src[y0, x0] + ax * (1 - ay) * src[y0, x0 + 1]
                      + (1 - ax) * ay * src[y0 + 1, x0] + ax * ay * src[y0 + 1, x0 + 1])
-                rs.append(ref[min(max(y + dy, 0), H - 1), min(max(x + dx, 0), W - 1)] / 255.0)
+                rs.append(float(ref[min(max(y + dy, 0), H - 1), min(max(x + dx, 0), W - 1)]) / unit)
                 ss.append(s)
         c = len(rs)
         if c < k // 2:
@@ -291,6 +293,51 @@ def test_cost_fp64_matches_brute_force_stride2_and_topk(orc, scenes):
                 b = brute_cost(job.ref, job.srcs, job.K, job.cams, x, y, job.gt_depth[y, x],
                                job.gt_normal[y, x], 5, 2, top_k)
                 assert c == pytest.approx(b, abs=1e-9)
+
+
+def _float_gray(img, seed):
+    """Float gray images with sub-DN detail that 8-bit levels cannot hold."""
+    rng = np.random.default_rng(seed)
+    return ((img.astype(np.float64) + rng.uniform(0.0, 1.0, img.shape)) / 256.0).astype(np.float32)
+
+
+def test_float_images_cost_fp64_matches_brute_force(orc, scenes):
+    """The oracle's float-image planes (gray given directly) against the
+    independent evaluation on the same float32 values."""
+    job = scenes.sphere_job(48, 36, 3, scene_seed=13, noise_sigma=2.0, radius=3, stride=1,
+                            iterations=0)
+    ref = _float_gray(job.ref, 1)
+    srcs = [_float_gray(s, 2 + i) for i, s in enumerate(job.srcs)]
+    v = orc.View(ref, srcs, job.Kinv, job.A, job.b, job.dmin, job.dmax, radius=3, stride=1)
+    v8 = oracle_view(orc, job)
+    rng = np.random.default_rng(10)
+    d0, n0 = v8.init_fp32(7)
+    checked = moved = 0
+    for _ in range(60):
+        x, y = int(rng.integers(0, 48)), int(rng.integers(0, 36))
+        for d, n in ((job.gt_depth[y, x], job.gt_normal[y, x]), (d0[y, x], n0[y, x])):
+            c, flags = v.cost_fp64(x, y, d, n)
+            if flags:
+                continue
+            b = brute_cost(ref, srcs, job.K, job.cams, x, y, d, n, 3, 1, 3)
+            assert c == pytest.approx(b, abs=1e-9), (x, y, d, n)
+            c8, f8 = v8.cost_fp64(x, y, d, n)
+            moved += (not f8) and abs(c - c8) > 1e-6   # the float planes are the ones read
+            checked += 1
+    assert checked > 80 and moved > checked // 2
+
+
+def test_float_view_of_8bit_levels_runs_like_the_8bit_view(orc, c1):
+    """gray = DN / 255 rounded to float32, given as float planes, is what the
+    oracle forms from 8-bit planes itself: the FP32 run is bit-identical."""
+    v8 = oracle_view(orc, c1)
+    g = lambda a: (a.astype(np.float32) / np.float32(255.0)).astype(np.float32)
+    vf = orc.View(g(c1.ref), [g(s) for s in c1.srcs], c1.Kinv, c1.A, c1.b, c1.dmin, c1.dmax,
+                  radius=c1.radius, stride=c1.stride, top_k=c1.top_k)
+    a = v8.patchmatch_fp32(1, 5, threads=4)
+    b = vf.patchmatch_fp32(1, 5, threads=4)
+    for x, y in zip(a, b):
+        assert np.array_equal(x, y)
 
 
 def test_cost_fp32_reference_order_within_its_rounding(orc, c1):
