@@ -11,6 +11,9 @@ the tap loop computes it: two FMAs per coordinate, a reciprocal off by up to
 one ulp either way (v_rcp_f32), a rounded product, floor.  Every tap of an
 accepted window must be valid (|w_z| >= 1e-8, 0 <= floor(u) <= W-2,
 0 <= floor(v) <= H-2); rejected windows are merely sent to the masked loop.
+The same holds for `z_clear`, which lets the FP16-storage masked loop drop
+its per-tap |w_z| >= 1e-8 test: every window it accepts, of w_z terms that
+nearly cancel at the guard, has |w_z| >= 1e-8 at every tap.
 """
 import numpy as np
 
@@ -167,3 +170,75 @@ def test_non_finite_windows_are_rejected():
     h = np.tile(np.array([[1, 0], [0, 1], [0, 0]], f32), (3, 1, 1))
     with np.errstate(invalid="ignore"):
         assert not window_inside(c, h, 5, 100, 100).any()
+
+
+# ---------------------------------------------------------------------------
+# z_clear: the FP16-storage masked loop drops its per-tap |w_z| >= 1e-8 test
+# when this closed-form bound holds on every lane.
+# ---------------------------------------------------------------------------
+def z_clear(c, h, R, eps=EPS):
+    """FP32 restatement of z_clear (pm_pixel.h)."""
+    cz, h20, h21 = c[:, 2], h[:, 2, 0], h[:, 2, 1]
+    R = f32(R)
+    e = f32(eps)
+    az = (np.abs(h20) + np.abs(h21)).astype(f32)
+    Tz = fma(R, az, np.abs(cz))
+    return (fma(-e, Tz, fma(-R, az, cz)) >= f32(1e-8)) | (fma(e, Tz, fma(R, az, cz)) <= f32(-1e-8))
+
+
+def taps_wz_clear(c, h, NT, ST):
+    """|w_z| >= 1e-8 for every tap, evaluated as the tap loop does."""
+    R = (NT - 1) * ST // 2
+    ok = np.ones(len(c), bool)
+    for j in range(NT):
+        bz = fma(h[:, 2, 1], f32(j * ST - R), c[:, 2])
+        for i in range(NT):
+            ok &= np.abs(fma(h[:, 2, 0], f32(i * ST - R), bz)) >= f32(1e-8)
+    return ok
+
+
+def _z_windows(rng, n):
+    """w_z near the 1e-8 guard somewhere in the window, of either sign, with
+    terms from 1e-8 to 1e3 that nearly cancel."""
+    R = 5.0
+    mag = 10.0 ** rng.uniform(-8, 3, n)
+    cz = mag * rng.choice([-1.0, 1.0], n)
+    # the extreme tap lands within a factor (1 +- 1e-6 .. 1e-1) of the guard
+    f = 1.0 + rng.choice([-1.0, 1.0], n) * 10.0 ** rng.uniform(-7, -1, n)
+    target = np.sign(cz) * 1e-8 * f
+    a = (cz - target) / R                      # |h20| + |h21| that puts the extreme at target
+    split = rng.uniform(0, 1, n)
+    h = np.zeros((n, 3, 2))
+    h[:, 2, 0] = a * split * rng.choice([-1.0, 1.0], n)
+    h[:, 2, 1] = a * (1 - split) * rng.choice([-1.0, 1.0], n)
+    c = np.zeros((n, 3))
+    c[:, 2] = cz
+    return c.astype(f32), h.astype(f32)
+
+
+def test_z_clear_windows_have_no_tap_near_the_guard():
+    rng = np.random.default_rng(21)
+    accepted = rejected_fine = 0
+    for NT, ST in ((6, 2), (11, 1), (3, 1)):
+        R = (NT - 1) * ST // 2
+        c, h = _z_windows(rng, 200000)
+        clear = z_clear(c, h, R)
+        fine = taps_wz_clear(c, h, NT, ST)
+        assert fine[clear].all(), (NT, ST, int((~fine[clear]).sum()))
+        accepted += int(clear.sum())
+        rejected_fine += int((fine & ~clear).sum())
+    assert accepted > 100000                     # not vacuous
+    print("z_clear: accepted", accepted, "rejected although clear", rejected_fine)
+
+
+def test_z_clear_check_detects_a_missing_margin():
+    """Without the 2^-21 margin the same bound accepts windows with a tap
+    below the guard."""
+    rng = np.random.default_rng(21)
+    bad = 0
+    for NT, ST in ((6, 2), (11, 1)):
+        R = (NT - 1) * ST // 2
+        c, h = _z_windows(rng, 200000)
+        clear = z_clear(c, h, R, eps=0.0)
+        bad += int((~taps_wz_clear(c, h, NT, ST)[clear]).sum())
+    assert bad > 0
