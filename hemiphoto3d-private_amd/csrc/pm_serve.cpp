@@ -313,12 +313,16 @@ void hp3d_pm_server::loop()
             });
             if (stop && order.empty() && active.empty()) break;
             // cancelled, failed and (at shutdown) every job leave once no
-            // step of theirs is in flight
+            // step of theirs is in flight and its image uploads are done (its
+            // arena block may go to a job with resident images only, whose
+            // steps do not wait for the upload stream)
             for (auto it = active.begin(); it != active.end();) {
                 auto& j = *it;
                 const bool fail =
                     j->dev_error || (j->d.fail_at_step > 0 && j->step >= j->d.fail_at_step);
-                if (j->inflight == 0 && (j->cancel || fail || stop)) {
+                const bool uploaded =
+                    j->ready || !j->ready_ev || hipEventQuery(j->ready_ev) != hipErrorNotReady;
+                if (j->inflight == 0 && uploaded && (j->cancel || fail || stop)) {
                     finish_job(j, fail ? HP3D_JOB_FAILED : HP3D_JOB_CANCELLED,
                                fail ? HP3D_ERR_DEVICE : HP3D_ERR_CANCELLED);
                     it = active.erase(it);
@@ -367,6 +371,8 @@ void hp3d_pm_server::loop()
                 for (auto& r : j->res) stats.image_reuses += r ? 1 : 0;
                 if (ue != hipSuccess) {
                     last_error = hipGetErrorString(ue);
+                    // what was queued of its upload runs before the block is reused
+                    (void)hipStreamSynchronize(upload);
                     finish_job(j, HP3D_JOB_FAILED, HP3D_ERR_DEVICE);
                     continue;
                 }

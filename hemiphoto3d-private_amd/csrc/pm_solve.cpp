@@ -222,6 +222,23 @@ int solve_views(hp3d_pm_ctx* ctx, int n, const hp3d_view_desc* d, const Image* c
         }
         int rc = bind_state(ctx, big);
         if (rc) return rc;
+        // likewise the buffers that caller hypotheses and SfM seeds pass
+        // through (a later view's import would otherwise free the buffer an
+        // earlier view's queued import kernel reads)
+        size_t in_px = 0, seed_bytes = 0, seed_px = 0;
+        for (int i = 0; i < n; ++i) {
+            const size_t px = (size_t)refs[i]->W * refs[i]->H;
+            if (d[i].init_depth) in_px = std::max(in_px, px);
+            if (d[i].num_seeds > 0) {
+                seed_bytes = std::max(seed_bytes, (size_t)d[i].num_seeds * sizeof(hp3d_seed_point));
+                seed_px = std::max(seed_px, px);
+            }
+        }
+        if (in_px) rc = ctx->grow(ctx->in_depth, in_px * sizeof(float), "input depth");
+        if (!rc && in_px) rc = ctx->grow(ctx->in_normal, 3 * in_px * sizeof(float), "input normal");
+        if (!rc && seed_bytes) rc = ctx->grow(ctx->seed_pts, seed_bytes, "seed points");
+        if (!rc && seed_px) rc = ctx->grow(ctx->seed_owner, seed_px * sizeof(int), "seed owner map");
+        if (rc) return rc;
     }
     ctx->stats = hp3d_pm_stats{};
     ctx->mark(kPhaseStart);

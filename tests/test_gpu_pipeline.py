@@ -78,3 +78,42 @@ def test_run_views_rejects_bad_views(pkg, ctx, views):
         ctx.run_views([d, bad], [o, ctx.host_outputs(job.shape)])
     assert e.value.code == pkg.ERR_INVALID_ARG
     assert "view 1" in ctx.last_error()
+
+
+def _seeds(pkg, pts):
+    arr = (pkg.SeedPoint * len(pts))()
+    for i, (x, y, dep, n) in enumerate(pts):
+        arr[i].x, arr[i].y, arr[i].depth = x, y, dep
+        for k in range(3):
+            arr[i].normal[k] = n[k]
+    return arr, len(pts)
+
+
+def test_run_views_with_caller_hypotheses_and_seeds(pkg, ctx, views):
+    """Views that bring their own starting hypotheses or SfM seeds, in
+    growing sizes, so the buffers those pass through are the largest view's
+    from the start; each equals run_view on the same descriptor."""
+    descs, exps, shapes = [], [], []
+    for k in (1, 0, 2, 4, 2):                       # sizes down, up, down, up
+        job, d, _ = views[k]
+        H, W = job.shape
+        kw = dict(radius=5, stride=2, iterations=2, seed=job.seed + 7)
+        if k % 2 == 0:                               # start from another seed's hypotheses
+            d0, n0 = ctx.init_hypotheses(d, job.shape)
+            kw["init"] = (d0[::-1].copy(), n0[::-1].copy())
+        else:                                        # seeds painted over the random start
+            kw["seeds"] = _seeds(pkg, [(W // 3, H // 2, float(job.dmin + 1.0), (0.0, 0.0, -1.0)),
+                                       (2 * W // 3, H // 3, float(job.dmax - 1.0), (0.0, 0.6, -0.8))])
+            kw["seed_radius"] = 6
+        dd = pkg.make_desc(d.ref, [d.src[i] for i in range(d.num_src)], job.Kinv, job.A, job.b,
+                           job.dmin, job.dmax, **kw)
+        descs.append(dd)
+        shapes.append(job.shape)
+        exps.append(ctx.run_view(dd, job.shape))
+    outs = [ctx.host_outputs(s, pinned=True) for s in shapes]
+    ctx.run_views(descs, outs)
+    for o, e in zip(outs, exps):
+        _same(o, e)
+    for o in outs:
+        for a in o.values():
+            ctx.host_free(a)
