@@ -136,6 +136,10 @@ int hp3d::validate_desc(hp3d_pm_ctx* ctx, const hp3d_view_desc* d, const Image**
     for (int i = 0; i < 9; ++i)
         if (!std::isfinite(d->K_ref_inv[i]))
             return ctx->fail(HP3D_ERR_INVALID_ARG, "K_ref_inv is not finite");
+    for (int s = 0; s < d->num_src; ++s)
+        for (int i = 0; i < 12; ++i)
+            if (!std::isfinite(i < 9 ? d->A[s][i] : d->b[s][i - 9]))
+                return ctx->fail(HP3D_ERR_INVALID_ARG, "source %d: A or b is not finite", s);
     return HP3D_OK;
 }
 

@@ -579,6 +579,13 @@ int hp3d_server_submit(hp3d_pm_server* s, const hp3d_job_desc* d, uint64_t* id)
     for (int k = 0; k < d->num_src; ++k)
         if (!d->src[k] && !j->res[1 + k])
             return reject(HP3D_ERR_INVALID_ARG, "submit: missing source image");
+    for (int i = 0; i < 9; ++i)
+        if (!std::isfinite(d->K_ref_inv[i]))
+            return reject(HP3D_ERR_INVALID_ARG, "submit: K_ref_inv is not finite");
+    for (int k = 0; k < d->num_src; ++k)
+        for (int i = 0; i < 12; ++i)
+            if (!std::isfinite(i < 9 ? d->A[k][i] : d->b[k][i - 9]))
+                return reject(HP3D_ERR_INVALID_ARG, "submit: a source's A or b is not finite");
     // taps address an image plane with 32-bit buffer offsets and 24-bit row
     // offsets
     if (!image_dims_supported(j->d.width, j->d.height, s->o.store))

@@ -256,3 +256,23 @@ def test_widest_row_pitch_accepted_and_solved(pkg, ctx):
     inner = out["cost"][R:H - R - 1, R:W - R - 1]
     print(f"max cost inside {inner.max():.2e}", flush=True)
     assert inner.max() < 1e-5
+
+
+def test_non_finite_homography_terms_refused(pkg, ctx, scenes):
+    """A source whose A or b holds a NaN or an infinity is refused before
+    any launch (by the per-view, multi-view and batched entry points)."""
+    job = scenes.sphere_job(64, 48, 3, scene_seed=3, radius=2, stride=1, iterations=1)
+    ref = ctx.upload(job.ref)
+    srcs = [ctx.upload(s) for s in job.srcs]
+    for which, val in (("A", float("nan")), ("b", float("inf"))):
+        A = [np.array(a, dtype=np.float64) for a in job.A]
+        b = [np.array(x, dtype=np.float64) for x in job.b]
+        (A if which == "A" else b)[1].flat[0] = val
+        d = pkg.make_desc(ref, srcs, job.Kinv, A, b, job.dmin, job.dmax, radius=2, stride=1,
+                          iterations=1)
+        with pytest.raises(pkg.HP3DError) as e:
+            ctx.run_view(d, job.shape)
+        assert e.value.code == pkg.ERR_INVALID_ARG and "not finite" in str(e.value)
+        with pytest.raises(pkg.HP3DError) as e:
+            ctx.batch_run([d], [job.shape])
+        assert e.value.code == pkg.ERR_INVALID_ARG

@@ -145,6 +145,11 @@ def test_invalid_submissions(pkg, serve_jobs):
             with pytest.raises(pkg.HP3DError) as e:
                 srv.submit(j.ref, j.srcs, j.Kinv, j.A, j.b, 2.0, dmax, iters)
             assert e.value.code == pkg.ERR_INVALID_ARG, (dmax, iters)
+        bad_b = [np.array(b, dtype=np.float64) for b in j.b]
+        bad_b[-1][2] = float("nan")
+        with pytest.raises(pkg.HP3DError) as e:
+            srv.submit(j.ref, j.srcs, j.Kinv, j.A, bad_b, 2.0, 12.0, 1)
+        assert e.value.code == pkg.ERR_INVALID_ARG
         with pytest.raises(pkg.HP3DError):
             srv.poll(123456)
 
