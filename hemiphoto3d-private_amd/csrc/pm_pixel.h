@@ -14,6 +14,8 @@
 
 #include "pm_device.h"
 
+// Build-time switch of the first iteration's depth-ordered candidate list in
+// the half-pass (1: on, the default; 0: owner order in every iteration).
 #ifndef HP3D_BUCKET
 #define HP3D_BUCKET 1
 #endif
