@@ -261,7 +261,7 @@ __device__ __forceinline__ WinRef window_ref_sums(const float* __restrict__ lwin
 // to 1.2e-3 of cost error on small views.  Shifted, the summands are of the
 // size of the spread.  When every tap is valid both loops take the first
 // tap's sample, so the variants still agree bit for bit.
-template <int NT, int ST, int STORE, bool MASKED, bool ZCHK = true>
+template <int NT, int ST, int STORE, bool MASKED, bool ZCHK = true, int RU = (NT >= 9 ? 2 : 1)>
 __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
                                            const float* __restrict__ lwin, int lp, float cx,
                                            float cy, float cz, float h00, float h01, float h10,
@@ -273,11 +273,12 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
     float s0 = 0.f;     // the shift: first valid sample of the window
     bool have = false;  // (masked loop) a valid tap has been seen
     int cnt = 0;
-    // window rows: unrolled by two for the 11-tap (stride 1) rows, where the
-    // extra loads in flight pay; one at a time for the 6-tap rows, where
-    // unrolling measured slower
-    constexpr int kRowUnroll = NT >= 9 ? 2 : 1;
-#pragma unroll kRowUnroll
+    // window rows (RU at a time): by default two for the 11-tap (stride 1)
+    // rows of the half-pass, where the extra loads in flight pay, one for the
+    // 6-tap rows, where unrolling measured slower; the initial-cost pass takes
+    // its 11-tap rows one at a time too (its random hypotheses scatter the
+    // gathers: 0.751 -> 0.442 ms per colour with FP32 storage at 1600x1200)
+#pragma unroll RU
     for (int j = 0; j < NT; ++j) {
         const float oy = (float)(j * ST - R);
         const float bx = fmaf(h01, oy, cx);
@@ -387,7 +388,7 @@ __device__ __forceinline__ bool z_clear(float cz, float h20, float h21, float R)
 // in FP32 with |dx|, |dy| <= 5, so the coordinates are accurate to the FP32
 // representation of w even for grazing planes.
 // ---------------------------------------------------------------------------
-template <int NT, int ST, int STORE>
+template <int NT, int ST, int STORE, int RU = (NT >= 9 ? 2 : 1)>
 __device__ __forceinline__ float cand_cost(const DevView& v, const float* __restrict__ lwin,
                                            int lp, const WinRef& wr, int x, int y, double rx,
                                            double ry, double rz, float d, float nx, float ny,
@@ -435,16 +436,16 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
                                           Wq, Hq);
         TapSums m;
         if (__all(inside)) {
-            tap_window<NT, ST, STORE, false>(img, lwin, lp, cx, cy, cz, h00, h01, h10, h11, h20,
-                                             h21, umax, vmax, wr, m);
+            tap_window<NT, ST, STORE, false, true, RU>(img, lwin, lp, cx, cy, cz, h00, h01, h10,
+                                                       h11, h20, h21, umax, vmax, wr, m);
         } else if (STORE == kStoreF16 && __all(z_clear(cz, h20, h21, (float)R))) {
             // FP16 storage only: a third loop without the per-tap |w_z| test
             // measured +2 % there and -12 % with FP32 storage at stride 1
-            tap_window<NT, ST, STORE, true, false>(img, lwin, lp, cx, cy, cz, h00, h01, h10, h11,
-                                                   h20, h21, umax, vmax, wr, m);
+            tap_window<NT, ST, STORE, true, false, RU>(img, lwin, lp, cx, cy, cz, h00, h01, h10,
+                                                       h11, h20, h21, umax, vmax, wr, m);
         } else {
-            tap_window<NT, ST, STORE, true>(img, lwin, lp, cx, cy, cz, h00, h01, h10, h11, h20,
-                                            h21, umax, vmax, wr, m);
+            tap_window<NT, ST, STORE, true, true, RU>(img, lwin, lp, cx, cy, cz, h00, h01, h10,
+                                                      h11, h20, h21, umax, vmax, wr, m);
         }
         const int cnt = m.cnt;
         const float sr = m.sr, ss = m.ss, srr = m.srr, sss = m.sss, srs = m.srs;
@@ -550,8 +551,8 @@ __device__ __forceinline__ bool cost_pixel(const DevView& v, const float* tile, 
     const float4 h = StateIO<COH>::ld4(v.hyp[parity], p);
     const WinRef wr = window_ref_sums<NT, ST>(lwin, G::LP);
     StateIO<COH>::st1(v.cost[parity], p,
-                      cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x, y, rx, ry, rz, h.x, h.y,
-                                               h.z, h.w));
+                      cand_cost<NT, ST, STORE, 1>(v, lwin, G::LP, wr, x, y, rx, ry, rz, h.x, h.y,
+                                                  h.z, h.w));
     return true;
 }
 
@@ -632,7 +633,7 @@ __device__ __forceinline__ void cost_tile_sorted(const DevView& v, const float* 
         double rx, ry, rz;
         ray_fp64(v.kinv_d, x_o, y_o, rx, ry, rz);
         IO::st1(v.cost[parity], y_o * v.Wc + i_o,
-                cand_cost<NT, ST, STORE>(v, lwin, G::LP, wr, x_o, y_o, rx, ry, rz, hh.x, hh.y,
+                cand_cost<NT, ST, STORE, 1>(v, lwin, G::LP, wr, x_o, y_o, rx, ry, rz, hh.x, hh.y,
                                          hh.z, hh.w));
     }
 }

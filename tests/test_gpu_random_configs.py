@@ -1,7 +1,7 @@
 """Seeded random configurations of the plain path against the oracle (MI355X).
 
-112 small views from three fixed generator seeds (16 from one, 48 from each
-of two more): size (odd and even, down to a few windows across), 1..8
+304 small views from seven fixed generator seeds (16 from one, 48 from each
+of six more): size (odd and even, down to a few windows across), 1..8
 sources at random arc angles, window radius 1..5, stride 1 or 2, top-k 1..8,
 both storages, 1..3 iterations, and a first iteration of 0 or 3.  Each must
 give the oracle's random init bit for bit, costs of identical hypotheses
@@ -43,7 +43,7 @@ def _configs(n=16, seed=20261202):
     return out
 
 
-DEFAULT_SWEEP_SEEDS = (7, 8)
+DEFAULT_SWEEP_SEEDS = (7, 8, 9, 10, 11, 12)
 
 
 def _all_configs():
@@ -79,7 +79,9 @@ def test_random_config_matches_oracle(pkg, orc, ctx, scenes, cfg):
     c64, flags = v.cost_map_fp64(d0, n0)
     ok = flags == 0
     err = np.abs(c.astype(np.float64) - c64)[ok]
-    assert ok.mean() > 0.5
+    # enough unflagged pixels for the gate to mean something (small views
+    # with an 11x11 window have most taps near a border)
+    assert ok.sum() >= min(200, ok.size // 3), int(ok.sum())
     ys, xs = np.nonzero(ok)
     pick = np.random.default_rng(cfg["idx"]).choice(len(xs), min(len(xs), 1500), replace=False)
     ref32 = max(abs(v.cost_fp32(int(xs[i]), int(ys[i]), float(d0[ys[i], xs[i]]),
