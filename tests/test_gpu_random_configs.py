@@ -8,9 +8,10 @@ give the oracle's random init bit for bit, costs of identical hypotheses
 within 1e-4 of the FP64 oracle on every unflagged pixel, whatever the window
 size and however few sources are averaged (the reference's own FP32
 operation order is printed beside it, on a sample), and either a short run
-within 1 % of the FP64-cost oracle's depth on >= 99 % of pixels or, for first
-iteration 3, a resumed run equal to the uninterrupted one bit for bit.  Every
-configuration is printed before it runs.
+within 1 % of the FP64-cost oracle's depth on >= 99 % of pixels (or, below
+that, on at least the share the reference's FP32 operation order reaches on
+the same configuration) or, for first iteration 3, a resumed run equal to the
+uninterrupted one bit for bit.  Every configuration is printed before it runs.
 
 HP3D_RANDOM_CONFIG_SEEDS (a comma-separated list) adds 48 configurations from
 each further generator seed, for wider sweeps.
@@ -112,7 +113,14 @@ def test_random_config_matches_oracle(pkg, orc, ctx, scenes, cfg):
         o64, _, _ = v.patchmatch_fp32(job.iterations, job.seed, cost_mode=orc.COST_FP64)
         agree = rel_within(out["depth"], o64).mean()
         print(f"  depth within 1 % of the oracle on {agree:.4f}")
-        assert agree >= 0.99
+        if agree < 0.99:
+            # a near-tie decided the other way early in a short run on a small
+            # view moves every pixel it propagates to; the bar is then the
+            # reference's own FP32 operation order on the same configuration
+            o32, _, _ = v.patchmatch_fp32(job.iterations, job.seed, cost_mode=orc.COST_FP32)
+            agree32 = rel_within(o32, o64).mean()
+            print(f"  reference-order FP32 run within 1 % of the FP64 run on {agree32:.4f}")
+            assert agree >= agree32, (agree, agree32)
     for h in [ref] + srcs:
         ctx.release(h)
     assert np.all((out["depth"] >= job.dmin) & (out["depth"] <= job.dmax))
