@@ -82,21 +82,36 @@ template <> __device__ __forceinline__ float load_plain<kStoreF16>(const void* p
     return (float)static_cast<const _Float16*>(p)[off];
 }
 
-// Source planes are read through raw buffer resources: the hardware
-// range-checks every access against the plane's size and returns 0 outside
-// it, so a tap whose computed offset is wild (a lane that will be masked) can
-// never fault.  Dword 3 = 0x00020000 is the raw 32-bit data format of
-// gfx9-family buffer descriptors.
+// Planes are read through buffer resources: the hardware range-checks every
+// access and returns 0 outside the plane, so a tap whose computed address is
+// wild (a lane that will be masked) can never fault.  Dword 3 = 0x00020000 is
+// the raw 32-bit data format of gfx9-family buffer descriptors.
+//   stride 0 (raw): num_records counts bytes and the load takes a byte
+//     offset, checked against it;
+//   stride s (structured, the source gather planes): num_records counts
+//     records of s bytes and the load takes a record index, which the buffer
+//     unit checks against num_records and scales by s.  The check is on the
+//     index alone: a 16-byte load at an 8-byte stride returns the record and
+//     its successor in full (measured on gfx950).
 constexpr int kBufferRsrcWord3 = 0x00020000;
-// An offset past every plane (planes hold at most kMaxPlaneBytes): the load
-// is range-checked away and returns zeros.
-constexpr unsigned kOffNowhere = 0x7ffffff0u;
-static_assert(kOffNowhere >= kMaxPlaneBytes, "offset inside a plane");
 
-__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, int bytes)
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t make_rsrc(const void* p, int num_records, short stride = 0)
 {
-    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), (short)0, bytes, kBufferRsrcWord3);
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(p), stride, num_records, kBufferRsrcWord3);
 }
+
+typedef int v4i32_t __attribute__((ext_vector_type(4)));
+typedef int v2i32_t __attribute__((ext_vector_type(2)));
+// Structured buffer loads (the LLVM intrinsics; hipcc has no builtin for them).
+__device__ v4i32_t struct_load_b128(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                    int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v4i32");
+__device__ v2i32_t struct_load_b64(__amdgpu_buffer_rsrc_t rsrc, int vindex, int voffset, int soffset,
+                                   int aux) __asm("llvm.amdgcn.struct.ptr.buffer.load.v2i32");
+// A record index past every gather plane (a plane holds at most
+// kMaxPlaneBytes, so fewer records): the load is range-checked away and
+// returns zeros.
+constexpr unsigned kIdxNowhere = 0x7ffffff0u;
+static_assert(kIdxNowhere >= kMaxPlaneBytes, "index inside a plane");
 
 // Access to the hypothesis state planes (hyp, cost, change flags).
 // COH = false: ordinary loads and stores (one kernel launch per pass: the
@@ -180,17 +195,22 @@ template <> struct StateIO<true> {
 // of one per texel row, 1.6x at stride 2; the F16 plane's halved footprint
 // 1.13x more at stride 1; differences instead of row pairs, +2 % (F32) /
 // +5 % (F16) at stride 2 and +16 % at stride 1.)
+// A source's gather plane as records of one texel (FP32: 8 bytes, FP16: 4):
+// the load takes the record index iy * pitch + ix (one multiply-add per tap)
+// and reads that texel's record and its right neighbour's, (a, c-a, b, d-b).
+// A valid tap has ix <= W-2 and iy <= H-2, so both records lie inside the
+// plane; an invalid tap's index goes nowhere and reads zeros.
 template <int STORE> struct SrcImage;
 template <> struct SrcImage<kStoreF32> {
     __amdgpu_buffer_rsrc_t rsrc;
     unsigned pitch;
     __device__ SrcImage(const DevView& v, int s)
-        : rsrc(make_rsrc(v.src_coef[s], v.coef_bytes)), pitch((unsigned)v.pitch_plain) {}
+        : rsrc(make_rsrc(v.src_coef[s], v.coef_bytes / 8, 8)), pitch((unsigned)v.pitch_plain) {}
     __device__ __forceinline__ float sample(int ix, int iy, float au, float aw, bool ok = true) const
     {
-        unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 3;
-        off = ok ? off : kOffNowhere;
-        const float4 q = __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(rsrc, off, 0, 0));
+        unsigned idx = __umul24((unsigned)iy, pitch) + (unsigned)ix;
+        idx = ok ? idx : kIdxNowhere;
+        const float4 q = __builtin_bit_cast(float4, struct_load_b128(rsrc, (int)idx, 0, 0, 0));
         const float c0 = fmaf(aw, q.y, q.x), c1 = fmaf(aw, q.w, q.z);
         return fmaf(au, c1 - c0, c0);
     }
@@ -199,12 +219,12 @@ template <> struct SrcImage<kStoreF16> {
     __amdgpu_buffer_rsrc_t rsrc;
     unsigned pitch;
     __device__ SrcImage(const DevView& v, int s)
-        : rsrc(make_rsrc(v.src_coef[s], v.coef_bytes)), pitch((unsigned)v.pitch_plain) {}
+        : rsrc(make_rsrc(v.src_coef[s], v.coef_bytes / 4, 4)), pitch((unsigned)v.pitch_plain) {}
     __device__ __forceinline__ float sample(int ix, int iy, float au, float aw, bool ok = true) const
     {
-        unsigned off = (__umul24((unsigned)iy, pitch) + (unsigned)ix) << 2;
-        off = ok ? off : kOffNowhere;
-        const half4_t q = __builtin_bit_cast(half4_t, __builtin_amdgcn_raw_buffer_load_b64(rsrc, off, 0, 0));
+        unsigned idx = __umul24((unsigned)iy, pitch) + (unsigned)ix;
+        idx = ok ? idx : kIdxNowhere;
+        const half4_t q = __builtin_bit_cast(half4_t, struct_load_b64(rsrc, (int)idx, 0, 0, 0));
         const float c0 = fmaf(aw, (float)q.y, (float)q.x), c1 = fmaf(aw, (float)q.w, (float)q.z);
         return fmaf(au, c1 - c0, c0);
     }
