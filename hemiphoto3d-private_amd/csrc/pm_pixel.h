@@ -14,8 +14,9 @@
 
 #include "pm_device.h"
 
-// Build-time switch of the first iteration's depth-ordered candidate list in
-// the half-pass (1: on, the default; 0: owner order in every iteration).
+// Build-time choice of the half-pass's depth-ordered candidate list: the
+// number of leading iterations of a solve that order a wave's candidates by
+// depth (1, the default: the first; 0: owner order in every iteration).
 #ifndef HP3D_BUCKET
 #define HP3D_BUCKET 1
 #endif
@@ -764,7 +765,7 @@ __device__ __forceinline__ void halfpass_pixel(const DevView& v, const float* ti
         total += (unsigned)__popcll(m);
     }
 #if HP3D_BUCKET
-    if (NT < 9 && iteration == v.first_iteration) {
+    if (NT < 9 && iteration < v.first_iteration + HP3D_BUCKET) {
         // First iteration: the neighbours hold random, unrelated depths, so
         // owner order does not cluster the gathers.  Order the items by
         // inverse depth (disparity) instead, in kBuckets buckets (a counting
