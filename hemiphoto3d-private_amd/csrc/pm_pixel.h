@@ -20,6 +20,20 @@
 #ifndef HP3D_BUCKET
 #define HP3D_BUCKET 1
 #endif
+// Build-time choice of the windows (by tap count) whose tap coordinates are
+// taken relative to an integer base near the window centre's source position
+// (default: up to 5x5 taps, the windows whose NCC a sub-pixel coordinate
+// error moves the most; 0: none; all windows cost 1-4 % at 36 and 121 taps).
+#ifndef HP3D_LOCAL_TAPS
+#define HP3D_LOCAL_TAPS 25
+#endif
+// Build-time choice of the window's tap order: 1 (default) = rows and
+// columns from the centre on (centre row, the rows below it, then the rows
+// above; the same for the columns of a row), so a window's first tap is its
+// centre; 0 = top-left first.
+#ifndef HP3D_CENTRE_FIRST
+#define HP3D_CENTRE_FIRST 1
+#endif
 
 namespace hp3d {
 
@@ -254,12 +268,15 @@ struct WinRef {
 template <int NT, int ST>
 __device__ __forceinline__ WinRef window_ref_sums(const float* __restrict__ lwin, int lp)
 {
+    constexpr int F = HP3D_CENTRE_FIRST ? NT / 2 : 0;   // first row and column
     float sr = 0.f, srr = 0.f;
 #pragma unroll 1
-    for (int j = 0; j < NT; ++j) {
+    for (int jr = 0; jr < NT; ++jr) {
+        const int j = jr + F < NT ? jr + F : jr + F - NT;
         const float* lrow = lwin + j * ST * lp;
 #pragma unroll
-        for (int i = 0; i < NT; ++i) {
+        for (int ir = 0; ir < NT; ++ir) {
+            const int i = ir + F < NT ? ir + F : ir + F - NT;
             const float r = lrow[i * ST];
             sr += r;
             srr = fmaf(r, r, srr);
@@ -281,15 +298,25 @@ __device__ __forceinline__ WinRef window_ref_sums(const float* __restrict__ lwin
 // carries over into n*sum(s^2) - sum(s)^2 amplified by mean^2 / variance: up
 // to 1.2e-3 of cost error on small views.  Shifted, the summands are of the
 // size of the spread.  When every tap is valid both loops take the first
-// tap's sample, so the variants still agree bit for bit.
+// tap's sample, so the variants still agree bit for bit.  The taps run from
+// the window centre on (window_ref_sums in the same order), so s0 is the
+// centre's sample: near the window mean, the running sums of s - s0 wander
+// instead of growing with the tap count, and sum(r (s - s0)), whose
+// reference values are not shifted, rounds 3-4x less over 121 taps.
+//
+// (cx, cy) and the columns (h00, h10), (h01, h11) may be relative to an
+// integer base (bu, bv): the taps' coordinates are then u - bu and v - bv,
+// and the base is added back to the texel index.
 template <int NT, int ST, int STORE, bool MASKED, bool ZCHK = true, int RU = (NT >= 9 ? 2 : 1)>
 __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
                                            const float* __restrict__ lwin, int lp, float cx,
                                            float cy, float cz, float h00, float h01, float h10,
                                            float h11, float h20, float h21, unsigned umax,
-                                           unsigned vmax, const WinRef& wr, TapSums& m)
+                                           unsigned vmax, const WinRef& wr, TapSums& m,
+                                           int bu = 0, int bv = 0)
 {
     constexpr int R = (NT - 1) * ST / 2;
+    constexpr int F = HP3D_CENTRE_FIRST ? NT / 2 : 0;   // first row and column
     float sr = 0.f, ss = 0.f, srr = 0.f, sss = 0.f, srs = 0.f;
     float s0 = 0.f;     // the shift: first valid sample of the window
     bool have = false;  // (masked loop) a valid tap has been seen
@@ -300,21 +327,25 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
     // its 11-tap rows one at a time too (its random hypotheses scatter the
     // gathers: 0.751 -> 0.442 ms per colour with FP32 storage at 1600x1200)
 #pragma unroll RU
-    for (int j = 0; j < NT; ++j) {
+    for (int jr = 0; jr < NT; ++jr) {
+        const int j = jr + F < NT ? jr + F : jr + F - NT;
         const float oy = (float)(j * ST - R);
         const float bx = fmaf(h01, oy, cx);
         const float by = fmaf(h11, oy, cy);
         const float bz = fmaf(h21, oy, cz);
         const float* lrow = lwin + j * ST * lp;
 #pragma unroll
-        for (int i = 0; i < NT; ++i) {
+        for (int ir = 0; ir < NT; ++ir) {
+            const int i = ir + F < NT ? ir + F : ir + F - NT;
             const float ox = (float)(i * ST - R);
             const float wx = fmaf(h00, ox, bx);
             const float wy = fmaf(h10, ox, by);
             const float wz = fmaf(h20, ox, bz);
             const float rw = __builtin_amdgcn_rcpf(wz);
             const float u = wx * rw, w = wy * rw;
-            const int ix = cvt_flr(u), iy = cvt_flr(w);
+            // (a wrapped sum lands far outside [0, umax], as any invalid tap)
+            const int ix = (int)((unsigned)bu + (unsigned)cvt_flr(u));
+            const int iy = (int)((unsigned)bv + (unsigned)cvt_flr(w));
             const float au = __builtin_amdgcn_fractf(u), aw = __builtin_amdgcn_fractf(w);
             // |w_z| guard and 0 <= u < W-1, 0 <= v < H-1 (:289-295); an
             // invalid tap's load goes nowhere and reads zeros (so sval = 0)
@@ -332,7 +363,7 @@ __device__ __forceinline__ void tap_window(const SrcImage<STORE>& img,
                 r = ok ? r : 0.0f;
                 cnt += ok ? 1 : 0;
             } else {
-                if (i == 0) s0 = j == 0 ? sraw : s0;
+                if (ir == 0) s0 = jr == 0 ? sraw : s0;
                 sval = sraw - s0;
             }
             if (MASKED) {
@@ -417,6 +448,7 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
 {
     constexpr int R = (NT - 1) * ST / 2;
     constexpr int kMinCount = (NT * NT) / 2;
+    constexpr bool kLocalCoords = NT * NT <= HP3D_LOCAL_TAPS;
     const double dd = d, dnx = nx, dny = ny, dnz = nz;
     const double nP = dd * fma(dnx, rx, fma(dny, ry, dnz * rz));
     if (fabs(nP) < 1e-8) return 2.0f;
@@ -450,6 +482,31 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
         const float h20 = (float)fma(b[2], g0, A[6]), h21 = (float)fma(b[2], g1, A[7]);
         const SrcImage<STORE> img(v, s);
 
+        // Small windows' taps relative to an integer base (bu, bv) near the
+        // centre's source position: the base comes out of the centre and the
+        // columns in FP64 (u - bu = (w_x - bu w_z) / w_z, still affine in the
+        // tap offset over w_z), so the taps' FP32 arithmetic works on u - bu,
+        // a few pixels, and rounds to ~ulp(R) instead of ~ulp(u) (1e-5 px at
+        // u ~ 100), an error a 3x3 window's NCC amplifies past 1e-4 of cost.
+        // A centre that projects beyond 2^22 pixels keeps the base 0.
+        float tx = cx, ty = cy, t00 = h00, t01 = h01, t10 = h10, t11 = h11;
+        int bu = 0, bv = 0;
+        if constexpr (kLocalCoords) {
+            const float rc = __builtin_amdgcn_rcpf(cz);
+            const float pu = cx * rc, pv = cy * rc;
+            bu = fabsf(pu) < 4194304.0f ? (int)rintf(pu) : 0;
+            bv = fabsf(pv) < 4194304.0f ? (int)rintf(pv) : 0;
+            const double nu = -(double)bu, nv = -(double)bv;
+            const double CZ = fma(b[2], gc, fma(A[6], X, fma(A[7], Y, A[8])));
+            const double H20 = fma(b[2], g0, A[6]), H21 = fma(b[2], g1, A[7]);
+            tx = (float)fma(nu, CZ, fma(b[0], gc, fma(A[0], X, fma(A[1], Y, A[2]))));
+            t00 = (float)fma(nu, H20, fma(b[0], g0, A[0]));
+            t01 = (float)fma(nu, H21, fma(b[0], g1, A[1]));
+            ty = (float)fma(nv, CZ, fma(b[1], gc, fma(A[3], X, fma(A[4], Y, A[5]))));
+            t10 = (float)fma(nv, H20, fma(b[1], g0, A[3]));
+            t11 = (float)fma(nv, H21, fma(b[1], g1, A[4]));
+        }
+
         // The unmasked tap loop when every lane's window is inside the
         // source image (windows with w_z < 0 throughout take the masked
         // loop, which gives the same result).
@@ -457,16 +514,16 @@ __device__ __forceinline__ float cand_cost(const DevView& v, const float* __rest
                                           Wq, Hq);
         TapSums m;
         if (__all(inside)) {
-            tap_window<NT, ST, STORE, false, true, RU>(img, lwin, lp, cx, cy, cz, h00, h01, h10,
-                                                       h11, h20, h21, umax, vmax, wr, m);
+            tap_window<NT, ST, STORE, false, true, RU>(img, lwin, lp, tx, ty, cz, t00, t01, t10,
+                                                       t11, h20, h21, umax, vmax, wr, m, bu, bv);
         } else if (STORE == kStoreF16 && __all(z_clear(cz, h20, h21, (float)R))) {
             // FP16 storage only: a third loop without the per-tap |w_z| test
             // measured +2 % there and -12 % with FP32 storage at stride 1
-            tap_window<NT, ST, STORE, true, false, RU>(img, lwin, lp, cx, cy, cz, h00, h01, h10,
-                                                       h11, h20, h21, umax, vmax, wr, m);
+            tap_window<NT, ST, STORE, true, false, RU>(img, lwin, lp, tx, ty, cz, t00, t01, t10,
+                                                       t11, h20, h21, umax, vmax, wr, m, bu, bv);
         } else {
-            tap_window<NT, ST, STORE, true, true, RU>(img, lwin, lp, cx, cy, cz, h00, h01, h10,
-                                                      h11, h20, h21, umax, vmax, wr, m);
+            tap_window<NT, ST, STORE, true, true, RU>(img, lwin, lp, tx, ty, cz, t00, t01, t10,
+                                                      t11, h20, h21, umax, vmax, wr, m, bu, bv);
         }
         const int cnt = m.cnt;
         const float sr = m.sr, ss = m.ss, srr = m.srr, sss = m.sss, srs = m.srs;
