@@ -1,7 +1,7 @@
 """Seeded random configurations of the plain path against the oracle (MI355X).
 
-304 small views from seven fixed generator seeds (16 from one, 48 from each
-of six more): size (odd and even, down to a few windows across), 1..8
+880 small views from nineteen fixed generator seeds (16 from one, 48 from
+each of eighteen more): size (odd and even, down to a few windows across), 1..8
 sources at random arc angles, window radius 1..5, stride 1 or 2, top-k 1..8,
 both storages, 1..3 iterations, and a first iteration of 0 or 3.  Each must
 give the oracle's random init bit for bit, costs of identical hypotheses
@@ -44,7 +44,7 @@ def _configs(n=16, seed=20261202):
     return out
 
 
-DEFAULT_SWEEP_SEEDS = (7, 8, 9, 10, 11, 12)
+DEFAULT_SWEEP_SEEDS = tuple(range(7, 25))
 
 
 def _all_configs():
