@@ -11,7 +11,9 @@ the tap loop computes it: two FMAs per coordinate, a reciprocal off by up to
 one ulp either way (v_rcp_f32), a rounded product, floor.  Every tap of an
 accepted window must be valid (|w_z| >= 1e-8, 0 <= floor(u) <= W-2,
 0 <= floor(v) <= H-2); rejected windows are merely sent to the masked loop.
-The same holds for `z_clear`, which lets the FP16-storage masked loop drop
+The same holds with the local tap coordinates of windows of up to 5x5 taps
+(an integer base taken out of the centre and columns in FP64).  The same
+holds for `z_clear`, which lets the FP16-storage masked loop drop
 its per-tap |w_z| >= 1e-8 test: every window it accepts, of w_z terms that
 nearly cancel at the guard, has |w_z| >= 1e-8 at every tap.
 """
@@ -170,6 +172,100 @@ def test_non_finite_windows_are_rejected():
     h = np.tile(np.array([[1, 0], [0, 1], [0, 0]], f32), (3, 1, 1))
     with np.errstate(invalid="ignore"):
         assert not window_inside(c, h, 5, 100, 100).any()
+
+
+# ---------------------------------------------------------------------------
+# Windows of up to 5x5 taps compute their taps relative to an integer base
+# (bu, bv) near the centre's source position: the base comes out of the
+# centre and the columns in FP64, the taps run on u - bu in FP32, and the
+# base is added back to floor(u - bu).  window_inside is still evaluated on
+# the absolute centre and columns; it must stay sound for that arithmetic.
+# ---------------------------------------------------------------------------
+def taps_valid_local(c, h, NT, ST, W, H, dbase=0):
+    """Every tap as the tap loop computes it with local coordinates (c, h
+    taken as the exact FP64 values the kernel rounds); dbase moves the base
+    by one either way, as a reciprocal 1 ulp off could."""
+    R = (NT - 1) * ST // 2
+    C, Hm = c.astype(np.float64), h.astype(np.float64)
+    with np.errstate(all="ignore"):
+        rc = (f32(1) / c[:, 2]).astype(f32)
+        pu, pv = (c[:, 0] * rc).astype(f32), (c[:, 1] * rc).astype(f32)
+        bu = np.where(np.abs(pu) < f32(4194304.0), np.rint(pu), 0).astype(np.int64) + dbase
+        bv = np.where(np.abs(pv) < f32(4194304.0), np.rint(pv), 0).astype(np.int64) + dbase
+    tx, ty = (C[:, 0] - bu * C[:, 2]).astype(f32), (C[:, 1] - bv * C[:, 2]).astype(f32)
+    t00, t01 = (Hm[:, 0, 0] - bu * Hm[:, 2, 0]).astype(f32), (Hm[:, 0, 1] - bu * Hm[:, 2, 1]).astype(f32)
+    t10, t11 = (Hm[:, 1, 0] - bv * Hm[:, 2, 0]).astype(f32), (Hm[:, 1, 1] - bv * Hm[:, 2, 1]).astype(f32)
+    ok = np.ones(len(c), bool)
+    for j in range(NT):
+        oy = f32(j * ST - R)
+        bx, by, bz = fma(t01, oy, tx), fma(t11, oy, ty), fma(h[:, 2, 1], oy, c[:, 2])
+        for i in range(NT):
+            ox = f32(i * ST - R)
+            wx, wy, wz = fma(t00, ox, bx), fma(t10, ox, by), fma(h[:, 2, 0], ox, bz)
+            with np.errstate(all="ignore"):
+                r = (f32(1) / wz).astype(f32)
+                for rr in (np.nextafter(r, f32(-np.inf)), np.nextafter(r, f32(np.inf))):
+                    ix = bu + np.floor((wx * rr).astype(f32))
+                    iy = bv + np.floor((wy * rr).astype(f32))
+                    ok &= (np.abs(wz) >= f32(1e-8)) & (ix >= 0) & (ix <= W - 2) & \
+                          (iy >= 0) & (iy <= H - 2)
+    return ok
+
+
+def _tight_windows(rng, n, W, H, R):
+    """Identity-scaled windows whose outermost tap is 1e-7 .. 1e-2 px from an
+    image border, in u or in v."""
+    d = 10.0 ** rng.uniform(-7, -2, n) * rng.choice([-1.0, 1.0], n)
+    low = rng.random(n) < 0.5
+    z = rng.uniform(0.5, 2.0, n)
+    t_u = np.where(low, R + d, (W - 1) - R - d)
+    t_v = np.where(low, R + d, (H - 1) - R - d)
+    in_v = rng.random(n) < 0.5
+    u = np.where(in_v, rng.uniform(R + 1, W - R - 2, n), t_u)
+    v = np.where(in_v, t_v, rng.uniform(R + 1, H - R - 2, n))
+    c = np.stack([u * z, v * z, z], 1)
+    h = np.zeros((n, 3, 2))
+    h[:, 0, 0] = h[:, 1, 1] = z
+    return c.astype(f32), h.astype(f32)
+
+
+def test_accepted_windows_have_only_valid_taps_with_local_coordinates():
+    rng = np.random.default_rng(3)
+    accepted = 0
+    for W, H in ((1600, 1200), (97, 61), (4000, 3000), (23, 17)):
+        for NT, ST in ((3, 1), (5, 1), (2, 2), (3, 2), (4, 2), (5, 2)):
+            R = (NT - 1) * ST // 2
+            if 2 * R + 3 > min(W, H):
+                continue
+            for kind in ("interior", "edge", "grazing", "tight"):
+                c, h = (_tight_windows(rng, 10000, W, H, R) if kind == "tight"
+                        else _windows(rng, 10000, W, H, kind))
+                ins = window_inside(c, h, R, W, H)
+                accepted += int(ins.sum())
+                for db in (0, -1, 1):
+                    ok = taps_valid_local(c[ins], h[ins], NT, ST, W, H, db)
+                    assert ok.all(), (W, H, NT, ST, kind, db, int((~ok).sum()))
+    assert accepted > 300000    # the test is not vacuous
+
+
+def test_local_coordinate_check_detects_missing_margins():
+    """Without window_inside's rounding margins, windows a sub-pixel fraction
+    from the border are accepted with taps the local arithmetic puts outside."""
+    global EPS, SHRINK
+    saved = EPS, SHRINK
+    EPS, SHRINK = 0.0, f32(1.0)
+    try:
+        rng = np.random.default_rng(3)
+        bad = 0
+        for W, H in ((1600, 1200), (4000, 3000)):
+            for NT, ST in ((3, 1), (5, 1), (4, 2)):
+                R = (NT - 1) * ST // 2
+                c, h = _tight_windows(rng, 20000, W, H, R)
+                ins = window_inside(c, h, R, W, H)
+                bad += int((~taps_valid_local(c[ins], h[ins], NT, ST, W, H)).sum())
+        assert bad > 100
+    finally:
+        EPS, SHRINK = saved
 
 
 # ---------------------------------------------------------------------------
